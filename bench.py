@@ -1,0 +1,247 @@
+"""Benchmark: Coulomb gauge fixing of a 48^3x96 lattice on one MI355X.
+
+A step is one full gauge fix to e2 < 1e-16 of a synthetic smooth
+configuration (eps 0.35, stored as FP32 SCIDAC input like the mixed-precision
+workload, converted to FP64 and reunitarized on the device), under the
+reference's own convergence-check schedule (estimate_sweeps_to_gf_progress
+1000), omega 1.95, hits 2, reunitarization every 10 sweeps.  Inputs are
+resident in HBM before the timed region (a device-side copy restores the
+unfixed configuration at the start of every step).
+
+value        site updates per second of the FP64 drop-in path
+               (half-sweeps x V/2 / wall time of the step)
+fp64_seconds wall time to e2 < 1e-16, FP64 path
+mixed_*      the FP32 -> FP64 path on the same input and schedule
+roofline     the FP64 sweep kernel: algorithmic bytes (6 spatial links + G(n),
+             read and written once: 2016 B per site update) over its mean
+             duration measured with HIP events on the engine's stream
+cpu_baseline the reference's OpenMP CPU path (compiled from its sources into
+             oracle/_ref) timed on this host on a bounded sample of the same
+             workload: a few half-sweeps of the same lattice
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gaugefix-coulomb_amd"))
+
+BYTES_PER_UPDATE_FP64 = 2016  # 14 x 144 B
+BYTES_PER_UPDATE_FP32 = 1008
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pmc_traffic(kernel_tag: str):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
+    """Times the reference's gaugefixOverrelaxation (oracle/_ref build) on n
+    half-sweeps of the same lattice; falls back to the C oracle port."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    vh = ns ** 3 * nt // 2
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_le64_mt.so")
+    Uw = U.copy()
+    G = np.zeros((U.shape[0], 3, 3), np.complex128)
+    dp = lambda a: a.ctypes.data_as(C.c_void_p)
+    if os.path.exists(ref_so):
+        lib = C.CDLL(ref_so)
+
+        class Generic(C.Structure):
+            _fields_ = [("tolerance", C.c_double), ("gfix_proxy", C.c_void_p),
+                        ("max_sweeps_to_fix", C.c_uint), ("estimate", C.c_uint),
+                        ("reunit", C.c_uint), ("error", C.c_bool)]
+
+        class ORParams(C.Structure):
+            _fields_ = [("generic_gf", Generic), ("omega_OR", C.c_float), ("hits", C.c_uint)]
+
+        lib.initGeometry.argtypes = [C.c_short, C.c_short]
+        lib.gaugefixOverrelaxation.argtypes = [C.c_void_p, C.c_void_p, ORParams]
+        lib.gaugefixOverrelaxation.restype = C.c_int
+        lib.reunitarizeField.argtypes = [C.c_void_p, C.c_uint]
+        lib.initGeometry(ns, nt)
+        lib.reunitarizeField(dp(Uw), 4 * ns ** 3 * nt)
+        e2fn = C.cast(lib.calculate_e2, C.c_void_p).value
+
+        def run(n):
+            p = ORParams(Generic(1e-300, e2fn, n, 100000000, 10, False), 1.95, 2)
+            t = time.perf_counter()
+            lib.gaugefixOverrelaxation(dp(Uw), dp(G), p)
+            return time.perf_counter() - t
+        kind = "reference"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        g = orc.geom(ns, nt)
+        orc.reunitarize(Uw)
+
+        def run(n):
+            p = orc.params(tolerance=1e-300, estimate=100000000, max_sweeps=n, reunit_every=10)
+            t = time.perf_counter()
+            orc.gaugefix(Uw, G, g, p)
+            return time.perf_counter() - t
+        kind = "port"
+    t1 = run(1)  # setup (G <- 1, initial e2) + one half-sweep
+    per = max(t1 * 0.5, 1e-3)
+    n = max(2, min(40, int(seconds / per)))
+    tn = run(1 + n)
+    per_sweep = max(tn - t1, 1e-9) / n
+    return {"value": vh / per_sweep, "unit": "site-updates/s", "cores": threads,
+            "kind": kind,
+            "sample": f"{n} half-sweeps of {ns}^3x{nt} (FP64, omega 1.95, hits 2) after one "
+                      f"warm-up call; time of the (1+{n})-sweep call minus the 1-sweep call"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ns", type=int, default=48)
+    ap.add_argument("--nt", type=int, default=96)
+    ap.add_argument("--eps", type=float, default=0.35)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this engine runs one lattice per GPU and the "
+              "boxes here have a single MI355X (see DESIGN.md for the timeslice sharding "
+              "plan); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import gfc
+    geom = gfc.Geometry(a.ns, a.nt)
+    V = geom.volume
+    vh = V // 2
+    log(f"generating {a.ns}^3x{a.nt} eps={a.eps} seed={a.seed}")
+    t = time.time()
+    U = gfc.generate_smooth(geom, a.seed, a.eps)
+    U32 = gfc.to_fp32(U)
+    log(f"  generated in {time.time() - t:.1f}s")
+    params = gfc.Params(tolerance=1e-16, estimate_sweeps_to_gf_progress=1000,
+                        sweeps_to_reunitarization=10, omega_OR=1.95, hits=2)
+    ctx = gfc.Context(0)
+    job = gfc.Job(ctx, geom, gfc.PREC_MIXED)
+    job.upload_config(U32)
+    job.reunitarize()
+    job.save_config()
+    plaq0 = job.plaquette()
+    fp64 = gfc.Options(precision=gfc.PREC_FP64)
+    mixed = gfc.Options(precision=gfc.PREC_MIXED)
+
+    for i in range(a.warmup):
+        job.restore_config()
+        rc, rep = job.fix(params, fp64)
+        log(f"warmup fp64: sweeps={rc} {rep.seconds:.3f}s")
+    sweeps = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        job.restore_config()
+        rc, rep = job.fix(params, fp64)
+        assert rc > 0, rc
+        sweeps.append(rc)
+    t1 = time.perf_counter()
+    step_s = (t1 - t0) / a.steps
+    value = sum(sweeps) * vh / (t1 - t0)
+    e2_64, th64, F64 = job.observables()
+    plaq64 = job.plaquette()
+    log(f"fp64: {a.steps} steps, sweeps={sweeps}, {step_s:.3f}s/step, {value:.4e} upd/s, "
+        f"e2={e2_64:.3e}")
+
+    # sweep-kernel duration with HIP events on the engine stream (untimed run)
+    job.restore_config()
+    rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1))
+    sweep_ms = rep.seconds_sweeps / rep.sweeps * 1e3
+    achieved = BYTES_PER_UPDATE_FP64 * vh / (sweep_ms * 1e-3) / 1e9
+    log(f"fp64 sweep kernel: {sweep_ms:.4f} ms/launch, {achieved:.1f} GB/s algorithmic")
+
+    out_mixed = {}
+    if not a.no_mixed:
+        job.restore_config()
+        job.fix(params, mixed)
+        ts, m_sweeps, m32 = [], [], []
+        for i in range(a.steps):
+            job.restore_config()
+            tm = time.perf_counter()
+            rc, rep = job.fix(params, mixed)
+            ts.append(time.perf_counter() - tm)
+            assert rc > 0, rc
+            m_sweeps.append(rc)
+            m32.append(rep.sweeps_fp32)
+        e2m, thm, Fm = job.observables()
+        plaqm = job.plaquette()
+        job.restore_config()
+        rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_MIXED, time_sweeps=1))
+        ms32 = None
+        mixed_s = float(np.mean(ts))
+        out_mixed = {
+            "mixed_seconds": round(mixed_s, 4),
+            "mixed_sweeps": m_sweeps[0], "mixed_sweeps_fp32": m32[0],
+            "mixed_speedup_vs_fp64": round(step_s / mixed_s, 3),
+            "mixed_final_e2": e2m, "mixed_F": Fm, "mixed_F_rel_diff_vs_fp64": abs(Fm - F64) / abs(F64),
+            "mixed_plaquette_drift": max(abs(plaqm[0] - plaq0[0]), abs(plaqm[1] - plaq0[1])),
+        }
+        log(f"mixed: {mixed_s:.3f}s sweeps={m_sweeps} fp32={m32} e2={e2m:.3e} "
+            f"speedup={step_s / mixed_s:.3f}")
+    job.close()
+    ctx.close()
+
+    cpu = None
+    if not a.no_cpu:
+        log("timing the CPU reference path ...")
+        cpu = cpu_baseline(U, a.ns, a.nt, a.cpu_seconds)
+        log(f"cpu: {cpu}")
+
+    rec = {
+        "metric": "site-updates/s & time to e2<1e-16, FP64 vs mixed, 48^3x96; serve p50/p99 ms",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded smooth SU(3) configuration, FP32 input)",
+        "config": {"workload": f"Coulomb gauge fix {a.ns}^3x{a.nt} eps {a.eps} to e2<1e-16, "
+                               "FP64 drop-in path, reference check schedule (est 1000), "
+                               "omega 1.95, hits 2, reunitarize every 10",
+                   "lattice": f"{a.ns}^3x{a.nt}", "sites": V, "parallelism": "single GPU"},
+        "fp64_seconds": round(step_s, 4),
+        "fp64_sweeps": sweeps[0],
+        "fp64_final_e2": e2_64,
+        "fp64_F": F64,
+        "fp64_plaquette_drift": max(abs(plaq64[0] - plaq0[0]), abs(plaq64[1] - plaq0[1])),
+        "sweep_kernel_ms": round(sweep_ms, 5),
+        **out_mixed,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic(f"sweep_fp64_{a.ns}x{a.nt}")},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,670 @@
+// engine.hip -- C ABI of the Coulomb gauge-fixing engine (see include/gfc.h).
+//
+// One gfc_ctx owns a HIP stream on one device; one gfc_job owns the device
+// fields of one lattice.  The driver loop of gfc_job_fix restates
+// gaugefixOverrelaxation (src/gauge_fixing.c:713-826 of the reference):
+// G <- 1, residue, check schedule, half-sweeps with alternating parity,
+// stall / convergence / max-sweeps tests, reunitarization every
+// sweeps_to_reunitarization sweeps and once at the end.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gfc.h"
+#include "kernels.hpp"
+
+using gfc::cplx;
+using gfc::kBlock;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(GFC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+struct gfc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *part = nullptr;  // per-block partial sums
+    size_t part_cap = 0;     // in blocks
+    double *res_dev = nullptr;
+    double *res_host = nullptr;  // pinned, 4 doubles
+    int *flag_dev = nullptr;
+};
+
+struct gfc_job {
+    gfc_ctx *ctx = nullptr;
+    gfc_geom geom{0, 0};
+    int64_t V = 0;
+    int Vh = 0;
+    int precision = GFC_PREC_FP64;
+    cplx<double> *cfg = nullptr;  // AoS FP64 U, 4V matrices
+    cplx<double> *u0 = nullptr;   // SoA spatial links, FP64
+    cplx<double> *g = nullptr;    // SoA G, FP64
+    cplx<float> *u0f = nullptr;   // SoA spatial links, FP32 (MIXED)
+    cplx<float> *gf = nullptr;    // SoA G, FP32 (MIXED)
+    cplx<double> *saved = nullptr;  // gfc_job_save_config copy of cfg
+};
+
+extern "C" {
+
+const char *gfc_last_error(void) { return g_last_error.c_str(); }
+
+const char *gfc_version(void) { return "gfc-mi355x 0.1 (gfx950)"; }
+
+void gfc_options_default(gfc_options *o) {
+    o->precision = GFC_PREC_FP64;
+    o->mixed_switch = 1e-9;
+    o->mixed_probe_every = 50;
+    o->verbose = 0;
+    o->time_sweeps = 0;
+}
+
+int gfc_ctx_create(int device, gfc_ctx **out) {
+    if (out == nullptr) return fail(GFC_EINVAL, "gfc_ctx_create: NULL output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(GFC_EDEVICE, "gfc_ctx_create: no HIP device available");
+    if (device < 0 || device >= n) return fail(GFC_EINVAL, "gfc_ctx_create: bad device index");
+    HIP_TRY(hipSetDevice(device));
+    gfc_ctx *c = new gfc_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->res_dev, 4 * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc(&c->res_host, 4 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&c->flag_dev, sizeof(int));
+    if (e != hipSuccess) {
+        gfc_ctx_destroy(c);
+        return fail(GFC_EDEVICE, std::string("gfc_ctx_create: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return GFC_OK;
+}
+
+void gfc_ctx_destroy(gfc_ctx *c) {
+    if (c == nullptr) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->part);
+    hipFree(c->res_dev);
+    hipHostFree(c->res_host);
+    hipFree(c->flag_dev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int64_t gfc_job_footprint(const gfc_geom *g, int precision) {
+    if (gfc_geom_check(g)) return -1;
+    const int64_t V = gfc_geom_volume(g);
+    int64_t b = V * 4 * 144      // cfg
+                + V * 3 * 144    // u0
+                + V * 144;       // g
+    if (precision == GFC_PREC_MIXED) b += V * 3 * 72 + V * 72;
+    return b;
+}
+
+void gfc_job_destroy(gfc_job *j) {
+    if (j == nullptr) return;
+    hipSetDevice(j->ctx->device);
+    hipStreamSynchronize(j->ctx->stream);
+    hipFree(j->cfg);
+    hipFree(j->u0);
+    hipFree(j->g);
+    hipFree(j->u0f);
+    hipFree(j->gf);
+    hipFree(j->saved);
+    delete j;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_partials(gfc_ctx *c, size_t blocks) {
+    if (blocks <= c->part_cap) return GFC_OK;
+    hipStreamSynchronize(c->stream);
+    hipFree(c->part);
+    c->part = nullptr;
+    c->part_cap = 0;
+    HIP_TRY(hipMalloc(&c->part, blocks * 3 * sizeof(double)));
+    c->part_cap = blocks;
+    return GFC_OK;
+}
+
+int set_identity(gfc_job *j) {
+    hipStream_t s = j->ctx->stream;
+    const unsigned b = nblocks(2 * (int64_t)j->Vh);
+    hipLaunchKernelGGL(gfc::k_g_identity<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh);
+    LAUNCH_CHECK();
+    if (j->gf) {
+        hipLaunchKernelGGL(gfc::k_g_identity<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh);
+        LAUNCH_CHECK();
+    }
+    return GFC_OK;
+}
+
+}  // namespace
+
+extern "C" int gfc_job_create(gfc_ctx *ctx, const gfc_geom *g, int precision, gfc_job **out) {
+    if (ctx == nullptr || out == nullptr) return fail(GFC_EINVAL, "gfc_job_create: NULL argument");
+    *out = nullptr;
+    if (gfc_geom_check(g))
+        return fail(GFC_EGEOMETRY, "gfc_job_create: ns and nt must be positive and even");
+    if (precision != GFC_PREC_FP64 && precision != GFC_PREC_MIXED)
+        return fail(GFC_EINVAL, "gfc_job_create: bad precision");
+    const int64_t V = gfc_geom_volume(g);
+    if (V / 2 > (int64_t)0x3fffffff)
+        return fail(GFC_EGEOMETRY, "gfc_job_create: lattice too large for 32-bit site indices");
+    HIP_TRY(hipSetDevice(ctx->device));
+    gfc_job *j = new gfc_job();
+    j->ctx = ctx;
+    j->geom = *g;
+    j->V = V;
+    j->Vh = (int)(V / 2);
+    j->precision = precision;
+    hipError_t e = hipMalloc(&j->cfg, (size_t)V * 4 * 9 * sizeof(cplx<double>));
+    if (e == hipSuccess) e = hipMalloc(&j->u0, (size_t)V * 3 * 9 * sizeof(cplx<double>));
+    if (e == hipSuccess) e = hipMalloc(&j->g, (size_t)V * 9 * sizeof(cplx<double>));
+    if (e == hipSuccess && precision == GFC_PREC_MIXED) {
+        e = hipMalloc(&j->u0f, (size_t)V * 3 * 9 * sizeof(cplx<float>));
+        if (e == hipSuccess) e = hipMalloc(&j->gf, (size_t)V * 9 * sizeof(cplx<float>));
+    }
+    if (e != hipSuccess) {
+        gfc_job_destroy(j);
+        return fail(GFC_ENOMEM, std::string("gfc_job_create: ") + hipGetErrorString(e));
+    }
+    int rc = ensure_partials(ctx, nblocks(V));
+    if (rc == GFC_OK) rc = set_identity(j);
+    if (rc != GFC_OK) {
+        gfc_job_destroy(j);
+        return rc;
+    }
+    *out = j;
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_upload_config(gfc_job *j, const void *U, int in_precision) {
+    if (j == nullptr || U == nullptr) return fail(GFC_EINVAL, "gfc_job_upload_config: NULL");
+    if (in_precision != 4 && in_precision != 8)
+        return fail(GFC_EINVAL, "gfc_job_upload_config: precision must be 4 or 8");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    hipStream_t s = j->ctx->stream;
+    const int64_t nreal = j->V * 4 * 18;
+    if (in_precision == 8) {
+        HIP_TRY(hipMemcpyAsync(j->cfg, U, nreal * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        // FP32 staging lands in the u0 planes (large enough: 27V complex doubles)
+        float *stage = reinterpret_cast<float *>(j->u0);
+        HIP_TRY(hipMemcpyAsync(stage, U, nreal * sizeof(float), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(gfc::k_f32_to_f64, dim3(nblocks(nreal)), dim3(kBlock), 0, s, stage,
+                           reinterpret_cast<double *>(j->cfg), nreal);
+        LAUNCH_CHECK();
+    }
+    int rc = set_identity(j);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_reunitarize(gfc_job *j) {
+    if (j == nullptr) return fail(GFC_EINVAL, "gfc_job_reunitarize: NULL");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    hipStream_t s = j->ctx->stream;
+    HIP_TRY(hipMemsetAsync(j->ctx->flag_dev, 0, sizeof(int), s));
+    const int64_t n = j->V * 4;
+    hipLaunchKernelGGL(gfc::k_reunit_aos, dim3(nblocks(n)), dim3(kBlock), 0, s, j->cfg, n,
+                       j->ctx->flag_dev);
+    LAUNCH_CHECK();
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, j->ctx->flag_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (flag) return fail(GFC_EPROJECT, "gfc_job_reunitarize: a link has a null row");
+    return GFC_OK;
+}
+
+namespace {
+
+// Reduces the per-block partials of the last observable kernel and returns the
+// three sums on the host.
+int finish_reduction(gfc_ctx *c, unsigned blocks, double out[3]) {
+    hipLaunchKernelGGL(gfc::k_reduce_final, dim3(1), dim3(kBlock), 0, c->stream, c->part,
+                       (int)blocks, c->res_dev);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(c->res_host, c->res_dev, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out[0] = c->res_host[0];
+    out[1] = c->res_host[1];
+    out[2] = c->res_host[2];
+    return GFC_OK;
+}
+
+// e2, theta, F of the factored state; fp32 selects the FP32 G (MIXED phase 1),
+// evaluated against the FP64 u0.
+int obs_factored(gfc_job *j, bool fp32, double out[3]) {
+    gfc_ctx *c = j->ctx;
+    const unsigned b = nblocks(2 * (int64_t)j->Vh);
+    if (fp32)
+        hipLaunchKernelGGL((gfc::k_obs_factored<double, float>), dim3(b), dim3(kBlock), 0,
+                           c->stream, j->u0, j->gf, j->geom.ns, j->Vh, c->part);
+    else
+        hipLaunchKernelGGL((gfc::k_obs_factored<double, double>), dim3(b), dim3(kBlock), 0,
+                           c->stream, j->u0, j->g, j->geom.ns, j->Vh, c->part);
+    LAUNCH_CHECK();
+    double s[3];
+    int rc = finish_reduction(c, b, s);
+    if (rc) return rc;
+    const double V = (double)j->V;
+    out[0] = s[0] / V;
+    out[1] = s[1] / (3.0 * V);
+    out[2] = s[2] / (9.0 * V);
+    return GFC_OK;
+}
+
+int obs_aos(gfc_job *j, double out[3]) {
+    gfc_ctx *c = j->ctx;
+    const unsigned b = nblocks(j->V);
+    hipLaunchKernelGGL(gfc::k_obs_aos, dim3(b), dim3(kBlock), 0, c->stream, j->cfg, j->geom.ns,
+                       j->V, c->part);
+    LAUNCH_CHECK();
+    double s[3];
+    int rc = finish_reduction(c, b, s);
+    if (rc) return rc;
+    const double V = (double)j->V;
+    out[0] = s[0] / V;
+    out[1] = s[1] / (3.0 * V);
+    out[2] = s[2] / (9.0 * V);
+    return GFC_OK;
+}
+
+int build_u0(gfc_job *j) {
+    hipStream_t s = j->ctx->stream;
+    const unsigned b = nblocks(2 * (int64_t)j->Vh);
+    hipLaunchKernelGGL(gfc::k_build_u0<double>, dim3(b), dim3(kBlock), 0, s, j->cfg, j->u0,
+                       j->geom.ns, j->Vh);
+    LAUNCH_CHECK();
+    if (j->u0f) {
+        hipLaunchKernelGGL(gfc::k_build_u0<float>, dim3(b), dim3(kBlock), 0, s, j->cfg, j->u0f,
+                           j->geom.ns, j->Vh);
+        LAUNCH_CHECK();
+    }
+    return GFC_OK;
+}
+
+int materialize(gfc_job *j) {
+    hipStream_t s = j->ctx->stream;
+    hipLaunchKernelGGL(gfc::k_materialize, dim3(nblocks(j->V)), dim3(kBlock), 0, s, j->cfg, j->g,
+                       j->geom.ns, j->geom.nt, j->Vh);
+    LAUNCH_CHECK();
+    return GFC_OK;
+}
+
+int half_sweep(gfc_job *j, int parity, bool fp32, double omega, int hits) {
+    hipStream_t s = j->ctx->stream;
+    const unsigned b = nblocks(j->Vh);
+    if (fp32)
+        hipLaunchKernelGGL(gfc::k_sweep<float>, dim3(b), dim3(kBlock), 0, s, j->u0f, j->gf,
+                           j->geom.ns, j->Vh, parity, (float)omega, hits, 0, j->Vh);
+    else
+        hipLaunchKernelGGL(gfc::k_sweep<double>, dim3(b), dim3(kBlock), 0, s, j->u0, j->g,
+                           j->geom.ns, j->Vh, parity, omega, hits, 0, j->Vh);
+    LAUNCH_CHECK();
+    return GFC_OK;
+}
+
+int reunit_g(gfc_job *j, bool fp32) {
+    hipStream_t s = j->ctx->stream;
+    const unsigned b = nblocks(2 * (int64_t)j->Vh);
+    if (fp32)
+        hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh);
+    else
+        hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh);
+    LAUNCH_CHECK();
+    return GFC_OK;
+}
+
+int promote_g(gfc_job *j) {
+    hipStream_t s = j->ctx->stream;
+    const unsigned b = nblocks(2 * (int64_t)j->Vh);
+    hipLaunchKernelGGL(gfc::k_g_promote, dim3(b), dim3(kBlock), 0, s, j->gf, j->g, j->Vh);
+    LAUNCH_CHECK();
+    return GFC_OK;
+}
+
+#define RC_TRY(expr)          \
+    do {                      \
+        int rc_ = (expr);     \
+        if (rc_) return rc_;  \
+    } while (0)
+
+struct SweepTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    hipStream_t s = nullptr;
+    ~SweepTimer() {
+        for (auto e : ev) hipEventDestroy(e);
+    }
+    void before() {
+        if (!on) return;
+        if (used + 2 > ev.size()) {
+            for (int k = 0; k < 64; k++) {
+                hipEvent_t e;
+                hipEventCreate(&e);
+                ev.push_back(e);
+            }
+        }
+        hipEventRecord(ev[used], s);
+    }
+    void after() {
+        if (!on) return;
+        hipEventRecord(ev[used + 1], s);
+        used += 2;
+    }
+    double seconds() {
+        if (!on) return 0;
+        hipStreamSynchronize(s);
+        double tot = 0;
+        for (size_t k = 0; k + 1 < used; k += 2) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            tot += ms;
+        }
+        return tot * 1e-3;
+    }
+};
+
+int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report *rep,
+            double *trace, int trace_len) {
+    if (j == nullptr || p == nullptr) return fail(GFC_EINVAL, "gfc_job_fix: NULL argument");
+    if (!gfc_params_valid(p)) return fail(GFC_EPARAMS, "gfc_job_fix: invalid parameters");
+    gfc_options o;
+    if (o_in)
+        o = *o_in;
+    else
+        gfc_options_default(&o);
+    if (o.precision == GFC_PREC_MIXED && j->precision != GFC_PREC_MIXED)
+        return fail(GFC_EINVAL, "gfc_job_fix: MIXED needs a job created with GFC_PREC_MIXED");
+    if (o.mixed_probe_every <= 0) o.mixed_probe_every = 50;
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    gfc_report r;
+    std::memset(&r, 0, sizeof r);
+
+    RC_TRY(build_u0(j));
+    RC_TRY(set_identity(j));
+    const double omega = (double)p->omega_OR;  // float in the reference
+    const int hits = (int)p->hits;
+    const int which = p->gfix_proxy == GFC_PROXY_THETA ? 1 : 0;
+    bool phase32 = o.precision == GFC_PREC_MIXED;
+    double ob[3];
+
+    SweepTimer timer;
+    timer.on = o.time_sweeps != 0;
+    timer.s = j->ctx->stream;
+
+    RC_TRY(obs_factored(j, false, ob));
+    r.residue_evals++;
+    double res = ob[which];
+    r.residue_initial = res;
+    if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", 0, res);
+    int due = gfc_next_check(0, res, p);
+    int sweep = 0;
+    int rc = 0;
+    bool done = false;
+    while (res > p->tolerance) {
+        timer.before();
+        RC_TRY(half_sweep(j, sweep & 1, phase32, omega, hits));
+        timer.after();
+        sweep++;
+        if (phase32) r.sweeps_fp32++;
+        if (trace && sweep <= trace_len) {
+            RC_TRY(obs_factored(j, phase32, ob));
+            trace[sweep - 1] = ob[which];
+        }
+        if (phase32 && sweep != due && sweep % o.mixed_probe_every == 0) {
+            RC_TRY(obs_factored(j, true, ob));
+            r.residue_evals++;
+            if (ob[which] <= o.mixed_switch) {
+                RC_TRY(promote_g(j));
+                phase32 = false;
+            }
+        }
+        if (sweep == due) {
+            const double last = res;
+            RC_TRY(obs_factored(j, phase32, ob));
+            r.residue_evals++;
+            res = ob[which];
+            if (phase32 && res <= o.mixed_switch) {
+                RC_TRY(promote_g(j));
+                phase32 = false;
+            }
+            if (std::fabs(res - last) / last < 10E-16) {
+                if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", sweep, res);
+                rc = GFC_ENOTCONVERGED;
+                done = true;
+                break;
+            } else if (res <= p->tolerance) {
+                break;
+            } else {
+                if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", sweep, res);
+                due = gfc_next_check((unsigned)sweep, res, p);
+            }
+        }
+        if ((unsigned)sweep >= p->max_sweeps_to_fix) {
+            rc = GFC_ENOTCONVERGED;
+            done = true;
+            break;
+        }
+        if (!((unsigned)sweep % p->sweeps_to_reunitarization)) RC_TRY(reunit_g(j, phase32));
+    }
+    if (phase32) {
+        // left the loop in the FP32 phase (tolerance reached there or max
+        // sweeps): hand the state over so that G and U are FP64
+        RC_TRY(promote_g(j));
+        phase32 = false;
+    }
+    if (!done) RC_TRY(reunit_g(j, false));
+    RC_TRY(materialize(j));
+    HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    r.seconds_sweeps = timer.seconds();
+    r.sweeps = sweep;
+    r.residue_final = res;
+    r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rep) *rep = r;
+    if (rc == GFC_ENOTCONVERGED) {
+        g_last_error = "gfc_job_fix: not converged (max sweeps or stalled residue)";
+        return rc;
+    }
+    return sweep;
+}
+
+}  // namespace
+
+extern "C" int gfc_job_fix(gfc_job *j, const gfc_params *p, const gfc_options *o,
+                           gfc_report *rep) {
+    return run_fix(j, p, o, rep, nullptr, 0);
+}
+
+extern "C" int gfc_job_fix_trace(gfc_job *j, const gfc_params *p, const gfc_options *o,
+                                 gfc_report *rep, double *trace, int trace_len) {
+    return run_fix(j, p, o, rep, trace, trace_len);
+}
+
+extern "C" int gfc_job_observables(gfc_job *j, double *e2, double *theta, double *F) {
+    if (j == nullptr) return fail(GFC_EINVAL, "gfc_job_observables: NULL");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    double ob[3];
+    RC_TRY(obs_aos(j, ob));
+    if (e2) *e2 = ob[0];
+    if (theta) *theta = ob[1];
+    if (F) *F = ob[2];
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_plaquette(gfc_job *j, double *spatial, double *temporal) {
+    if (j == nullptr) return fail(GFC_EINVAL, "gfc_job_plaquette: NULL");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    gfc_ctx *c = j->ctx;
+    const unsigned b = nblocks(j->V);
+    hipLaunchKernelGGL(gfc::k_plaquette_aos, dim3(b), dim3(kBlock), 0, c->stream, j->cfg,
+                       j->geom.ns, j->geom.nt, j->V, c->part);
+    LAUNCH_CHECK();
+    double s[3];
+    RC_TRY(finish_reduction(c, b, s));
+    if (spatial) *spatial = s[0] / (3.0 * (double)j->V);
+    if (temporal) *temporal = s[1] / (3.0 * (double)j->V);
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_download_config(gfc_job *j, double *U) {
+    if (j == nullptr || U == nullptr) return fail(GFC_EINVAL, "gfc_job_download_config: NULL");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    HIP_TRY(hipMemcpyAsync(U, j->cfg, (size_t)j->V * 4 * 18 * sizeof(double),
+                           hipMemcpyDeviceToHost, j->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_save_config(gfc_job *j) {
+    if (j == nullptr) return fail(GFC_EINVAL, "gfc_job_save_config: NULL");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    const size_t bytes = (size_t)j->V * 4 * 9 * sizeof(cplx<double>);
+    if (j->saved == nullptr) {
+        hipError_t e = hipMalloc(&j->saved, bytes);
+        if (e != hipSuccess) {
+            j->saved = nullptr;
+            return fail(GFC_ENOMEM, std::string("gfc_job_save_config: ") + hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(j->saved, j->cfg, bytes, hipMemcpyDeviceToDevice, j->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_restore_config(gfc_job *j) {
+    if (j == nullptr || j->saved == nullptr)
+        return fail(GFC_EINVAL, "gfc_job_restore_config: nothing saved");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    const size_t bytes = (size_t)j->V * 4 * 9 * sizeof(cplx<double>);
+    HIP_TRY(hipMemcpyAsync(j->cfg, j->saved, bytes, hipMemcpyDeviceToDevice, j->ctx->stream));
+    int rc = set_identity(j);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_get_link(gfc_job *j, int64_t site, int mu, double *out18) {
+    if (j == nullptr || out18 == nullptr || site < 0 || site >= j->V || mu < 0 || mu > 3)
+        return fail(GFC_EINVAL, "gfc_job_get_link: bad argument");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    HIP_TRY(hipMemcpyAsync(out18, j->cfg + (site * 4 + mu) * 9, 18 * sizeof(double),
+                           hipMemcpyDeviceToHost, j->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return GFC_OK;
+}
+
+extern "C" int gfc_job_download_gauge(gfc_job *j, double *G) {
+    if (j == nullptr || G == nullptr) return fail(GFC_EINVAL, "gfc_job_download_gauge: NULL");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    hipStream_t s = j->ctx->stream;
+    // AoS staging in the u0 planes, which the next fix rebuilds anyway
+    cplx<double> *stage = j->u0;
+    hipLaunchKernelGGL(gfc::k_g_to_aos, dim3(nblocks(j->V)), dim3(kBlock), 0, s, j->g, stage,
+                       j->geom.ns, j->Vh);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(G, stage, (size_t)j->V * 18 * sizeof(double), hipMemcpyDeviceToHost,
+                           s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GFC_OK;
+}
+
+extern "C" int gfc_fix_coulomb_or(gfc_ctx *ctx, const gfc_geom *g, const gfc_params *p,
+                                  const gfc_options *o, double *U, double *G, gfc_report *rep) {
+    if (ctx == nullptr || U == nullptr || G == nullptr || p == nullptr)
+        return fail(GFC_EINVAL, "gfc_fix_coulomb_or: NULL argument");
+    if (!gfc_params_valid(p)) return fail(GFC_EPARAMS, "gfc_fix_coulomb_or: invalid parameters");
+    const int prec = (o && o->precision == GFC_PREC_MIXED) ? GFC_PREC_MIXED : GFC_PREC_FP64;
+    gfc_job *j = nullptr;
+    RC_TRY(gfc_job_create(ctx, g, prec, &j));
+    int rc = gfc_job_upload_config(j, U, 8);
+    int sweeps = rc;
+    if (rc == GFC_OK) {
+        sweeps = gfc_job_fix(j, p, o, rep);
+        if (sweeps >= 0 || sweeps == GFC_ENOTCONVERGED) {
+            rc = gfc_job_download_config(j, U);
+            if (rc == GFC_OK) rc = gfc_job_download_gauge(j, G);
+            if (rc != GFC_OK) sweeps = rc;
+        }
+    }
+    gfc_job_destroy(j);
+    return sweeps;
+}
+
+extern "C" int gfc_reunitarize_field(gfc_ctx *ctx, double *field, int64_t n) {
+    if (ctx == nullptr || field == nullptr || n < 0)
+        return fail(GFC_EINVAL, "gfc_reunitarize_field: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    cplx<double> *d = nullptr;
+    HIP_TRY(hipMalloc(&d, (size_t)n * 9 * sizeof(cplx<double>) + 16));
+    int rc = GFC_OK;
+    int flag = 0;
+    do {
+        hipError_t e = hipMemcpyAsync(d, field, (size_t)n * 18 * sizeof(double),
+                                      hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemsetAsync(ctx->flag_dev, 0, sizeof(int), s);
+        if (e != hipSuccess) {
+            rc = fail(GFC_EDEVICE, hipGetErrorString(e));
+            break;
+        }
+        if (n > 0)
+            hipLaunchKernelGGL(gfc::k_reunit_aos, dim3(nblocks(n)), dim3(kBlock), 0, s, d, n,
+                               ctx->flag_dev);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(field, d, (size_t)n * 18 * sizeof(double), hipMemcpyDeviceToHost,
+                               s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&flag, ctx->flag_dev, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(GFC_EDEVICE, hipGetErrorString(e));
+    } while (0);
+    hipFree(d);
+    if (rc == GFC_OK && flag) rc = fail(GFC_EPROJECT, "gfc_reunitarize_field: null row");
+    return rc;
+}
+
+extern "C" int gfc_config_observables(gfc_ctx *ctx, const gfc_geom *g, const double *U,
+                                      double *e2, double *theta, double *F,
+                                      double *plaq_spatial, double *plaq_temporal) {
+    gfc_job *j = nullptr;
+    RC_TRY(gfc_job_create(ctx, g, GFC_PREC_FP64, &j));
+    int rc = gfc_job_upload_config(j, U, 8);
+    if (rc == GFC_OK) rc = gfc_job_observables(j, e2, theta, F);
+    if (rc == GFC_OK) rc = gfc_job_plaquette(j, plaq_spatial, plaq_temporal);
+    gfc_job_destroy(j);
+    return rc;
+}

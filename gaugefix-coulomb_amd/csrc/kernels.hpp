@@ -1,0 +1,495 @@
+// kernels.hpp -- HIP kernels of the Coulomb gauge-fixing engine (gfx950).
+//
+// Device state of one lattice (V = ns^3 nt sites, Vh = V/2 per parity):
+//   cfg : AoS FP64, the reference's U image (4V matrices, site-major, mu=T,Z,Y,X)
+//   u0  : SoA spatial links of the configuration being fixed, one plane of Vh
+//         complex values per (parity p, direction i = Z,Y,X, element k):
+//         plane index (p*3 + i)*9 + k
+//   g   : SoA gauge transformation, plane index p*9 + k
+// Within a parity the index is h = lex >> 1 with lex = ((t ns + z) ns + y) ns + x
+// (ns even), so the y/z neighbours of a site sit at a constant offset +-ns/2,
+// +-ns^2/2 in the other parity's planes and the x neighbours at h or h+-1:
+// every wavefront reads 64 consecutive complex values per plane.
+//
+// The sweep keeps the reference's state in factored form: the current links are
+// U_i(n) = G(n) u0_i(n) G(n+i)^dagger, so a half-sweep only rewrites G on the
+// active parity while u0 stays read-only.  w(n) = sum_i U_i(n) + U_i(n-i)^dagger
+// (src/gauge_fixing.c:493-531) becomes w(n) = G(n) W(n) with
+// W(n) = sum_i [u0_i(n) G(n+i)^dagger + u0_i(n-i)^dagger G(n-i)^dagger].
+#pragma once
+
+#include "su3.hpp"
+
+namespace gfc {
+
+constexpr int kBlock = 256;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (one L2 each); remap
+// so that each XCD walks one contiguous range of sites and the neighbour
+// reads of adjacent z-planes hit the same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
+    const unsigned nx = 8;
+    const unsigned q = nblocks / nx, r = nblocks % nx;
+    const unsigned x = bid % nx, i = bid / nx;
+    return x * q + (x < r ? x : r) + i;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_m(mat3<T> &m, const cplx<T> *__restrict__ plane0,
+                                       int64_t stride, int idx) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) m.e[k] = plane0[k * stride + idx];
+}
+
+template <typename T>
+__device__ __forceinline__ void store_m(const mat3<T> &m, cplx<T> *__restrict__ plane0,
+                                        int64_t stride, int idx) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) plane0[k * stride + idx] = m.e[k];
+}
+
+// Site geometry of parity index h on parity p.
+struct Site {
+    int x, y, z, t, xh, row;
+};
+
+__device__ __forceinline__ Site site_of(int h, int p, int ns) {
+    const int nh = ns >> 1;
+    Site s;
+    s.xh = h % nh;
+    int r = h / nh;
+    s.y = r % ns;
+    r /= ns;
+    s.z = r % ns;
+    s.t = r / ns;
+    s.x = 2 * s.xh + ((s.y + s.z + s.t + p) & 1);
+    s.row = h - s.xh;
+    return s;
+}
+
+// Parity indices (in the other parity) of the six spatial neighbours, in the
+// direction order i = 0 (Z), 1 (Y), 2 (X).
+struct Nbr {
+    int fwd[3], bwd[3];
+};
+
+__device__ __forceinline__ Nbr neighbours(const Site &s, int h, int ns) {
+    const int nh = ns >> 1, sz = ns * nh;
+    Nbr n;
+    n.fwd[0] = h + (s.z + 1 == ns ? -(ns - 1) * sz : sz);
+    n.bwd[0] = h + (s.z == 0 ? (ns - 1) * sz : -sz);
+    n.fwd[1] = h + (s.y + 1 == ns ? -(ns - 1) * nh : nh);
+    n.bwd[1] = h + (s.y == 0 ? (ns - 1) * nh : -nh);
+    n.fwd[2] = s.row + ((s.x + 1 == ns ? 0 : s.x + 1) >> 1);
+    n.bwd[2] = s.row + ((s.x == 0 ? ns - 1 : s.x - 1) >> 1);
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// One red/black half-sweep on parity p over parity indices [h0, h0 + count).
+// Reads 6 u0 links, 6 neighbour G and G(n); writes G(n).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0,
+                                                  cplx<T> *__restrict__ g, int ns, int Vh,
+                                                  int p, T omega, int hits, int h0,
+                                                  int count) {
+    const int tid = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+    if (tid >= count) return;
+    const int h = h0 + tid;
+    const Site s = site_of(h, p, ns);
+    const Nbr nb = neighbours(s, h, ns);
+    const int64_t st = Vh;
+    const cplx<T> *gq = g + (int64_t)(1 - p) * 9 * st;  // other parity's G
+    mat3<T> W;
+    m_zero(W);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        mat3<T> uf, gf, ub, gb;
+        load_m(uf, u0 + (int64_t)(p * 3 + i) * 9 * st, st, h);
+        load_m(gf, gq, st, nb.fwd[i]);
+        load_m(ub, u0 + (int64_t)((1 - p) * 3 + i) * 9 * st, st, nb.bwd[i]);
+        load_m(gb, gq, st, nb.bwd[i]);
+        m_mac_nd(W, uf, gf);  // u0_i(n) G(n+i)^dagger
+        m_mac_dd(W, ub, gb);  // u0_i(n-i)^dagger G(n-i)^dagger
+    }
+    cplx<T> *gp = g + (int64_t)p * 9 * st;
+    mat3<T> gn, w, upd, out;
+    load_m(gn, gp, st, h);
+    m_mul(gn, W, w);
+    la_or_update(w, hits, omega, upd);
+    m_mul(upd, gn, out);
+    store_m(out, gp, st, h);
+}
+
+// ---------------------------------------------------------------------------
+// G <- identity on both parities (setFieldToIdentity, src/fields.c:64-100).
+// The G kernels below take one thread per site over both parities:
+// idx in [0, 2 Vh), parity p = idx >= Vh, plane (p*9 + k) of length Vh.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_g_identity(cplx<T> *__restrict__ g, int Vh) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= 2 * Vh) return;
+    const int p = idx >= Vh, h = idx - p * Vh;
+    mat3<T> m;
+    m_identity(m);
+    store_m(m, g + (int64_t)p * 9 * Vh, Vh, h);
+}
+
+// Projects every G onto SU(3) (reunitarizeField on G, src/fields.c:144-187).
+// Only rows 0 and 1 are read: the projection recomputes row 2.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_g_reunit(cplx<T> *__restrict__ g, int Vh) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= 2 * Vh) return;
+    const int p = idx >= Vh, h = idx - p * Vh;
+    cplx<T> *base = g + (int64_t)p * 9 * Vh;
+    mat3<T> m;
+#pragma unroll
+    for (int k = 0; k < 6; k++) m.e[k] = base[(int64_t)k * Vh + h];
+    m_project_su3(m);
+    store_m(m, base, Vh, h);
+}
+
+// FP32 -> FP64 hand-off of G: promote then project in FP64.
+__global__ __launch_bounds__(kBlock) void k_g_promote(const cplx<float> *__restrict__ gf,
+                                                      cplx<double> *__restrict__ g, int Vh) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= 2 * Vh) return;
+    const int p = idx >= Vh, h = idx - p * Vh;
+    const cplx<float> *src = gf + (int64_t)p * 9 * Vh;
+    mat3<double> m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        cplx<float> v = src[(int64_t)k * Vh + h];
+        m.e[k] = c_make<double>(v.re, v.im);
+    }
+    m_project_su3(m);
+    store_m(m, g + (int64_t)p * 9 * Vh, Vh, h);
+}
+
+// cfg (AoS FP64) spatial links -> u0 planes of type T.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build_u0(const cplx<double> *__restrict__ cfg,
+                                                     cplx<T> *__restrict__ u0, int ns,
+                                                     int Vh) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= 2 * Vh) return;
+    const int p = idx >= Vh, h = idx - p * Vh;
+    const Site s = site_of(h, p, ns);
+    const int64_t lex = (int64_t)2 * s.row + s.x;
+    const int64_t st = Vh;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const cplx<double> *src = cfg + (lex * 4 + 1 + i) * 9;
+        cplx<T> *dst = u0 + (int64_t)(p * 3 + i) * 9 * st;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            cplx<double> v = src[k];
+            dst[k * st + h] = c_make<T>((T)v.re, (T)v.im);
+        }
+    }
+}
+
+// cfg <- G(n) cfg G(n+mu)^dagger for all four directions
+// (applyGaugeTransformationU, src/fields.c:375-435).
+__global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict__ cfg,
+                                                        const cplx<double> *__restrict__ g,
+                                                        int ns, int nt, int Vh) {
+    const int64_t lex = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (lex >= 2 * (int64_t)Vh) return;
+    const int x = (int)(lex % ns);
+    int64_t r = lex / ns;
+    const int y = (int)(r % ns);
+    r /= ns;
+    const int z = (int)(r % ns);
+    const int t = (int)(r / ns);
+    const int p = (x + y + z + t) & 1;
+    const int64_t st = Vh;
+    mat3<double> gn;
+    load_m(gn, g + (int64_t)p * 9 * st, st, (int)(lex >> 1));
+    const cplx<double> *gq = g + (int64_t)(1 - p) * 9 * st;
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+        int c[4] = {t, z, y, x};
+        c[mu] += 1;
+        if (c[mu] == (mu == 0 ? nt : ns)) c[mu] = 0;
+        const int64_t ln = (((int64_t)c[0] * ns + c[1]) * ns + c[2]) * ns + c[3];
+        mat3<double> gm, u, tmp, out;
+        load_m(gm, gq, st, (int)(ln >> 1));
+        cplx<double> *ptr = cfg + (lex * 4 + mu) * 9;
+#pragma unroll
+        for (int k = 0; k < 9; k++) u.e[k] = ptr[k];
+        m_mul(gn, u, tmp);
+        m_mul_nd(tmp, gm, out);
+#pragma unroll
+        for (int k = 0; k < 9; k++) ptr[k] = out.e[k];
+    }
+}
+
+// Projects n AoS FP64 matrices onto SU(3); sets *flag when a row was null
+// (reunitarizeField's -1).
+__global__ __launch_bounds__(kBlock) void k_reunit_aos(cplx<double> *__restrict__ m_,
+                                                       int64_t n, int *flag) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    cplx<double> *ptr = m_ + i * 9;
+    mat3<double> m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) m.e[k] = ptr[k];
+    if (!m_project_su3(m)) atomicOr(flag, 1);
+#pragma unroll
+    for (int k = 0; k < 9; k++) ptr[k] = m.e[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_f32_to_f64(const float *__restrict__ in,
+                                                       double *__restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+
+// SoA G (FP64) -> AoS image (site-major).
+__global__ __launch_bounds__(kBlock) void k_g_to_aos(const cplx<double> *__restrict__ g,
+                                                     cplx<double> *__restrict__ out, int ns,
+                                                     int Vh) {
+    const int64_t lex = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (lex >= 2 * (int64_t)Vh) return;
+    const int x = (int)(lex % ns);
+    int64_t r = lex / ns;
+    const int y = (int)(r % ns);
+    r /= ns;
+    const int z = (int)(r % ns);
+    const int t = (int)(r / ns);
+    const int p = (x + y + z + t) & 1;
+    mat3<double> m;
+    load_m(m, g + (int64_t)p * 9 * Vh, Vh, (int)(lex >> 1));
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[lex * 9 + k] = m.e[k];
+}
+
+// ---------------------------------------------------------------------------
+// Observables.  Per site: D = sum_i [(U_i(n) - U_i(n)^dag) - (U_i(n-i) - U_i(n-i)^dag)]
+// made traceless, div A = -(i/2) D (divergenceA/calculateA,
+// src/four_potential.c:35-113); e2 density = sum_a c_a^2 with the Gell-Mann
+// components of decomposeAlgebraSU3 (src/SU3_ops.c:1055-1107); theta density
+// = Re Tr(divA divA^dag) (calculateTheta, src/gauge_fixing.c:341-384); and
+// Re Tr sum_i U_i(n) for F (calculateF, :289-337).  All arithmetic in FP64.
+
+struct Obs {
+    double e2, theta, f;
+};
+
+__device__ __forceinline__ void accum_link(mat3<double> &D, const mat3<double> &u, double sgn) {
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const cplx<double> uab = u.e[3 * a + b], uba = u.e[3 * b + a];
+            D.e[3 * a + b].re += sgn * (uab.re - uba.re);
+            D.e[3 * a + b].im += sgn * (uab.im + uba.im);
+        }
+}
+
+__device__ __forceinline__ Obs site_obs(mat3<double> &D, double f) {
+    const double tr = (D.e[0].im + D.e[4].im + D.e[8].im) / 3.0;
+    D.e[0].im -= tr;
+    D.e[4].im -= tr;
+    D.e[8].im -= tr;
+    // A = -(i/2) D : A.re = D.im / 2, A.im = -D.re / 2
+    mat3<double> A;
+#pragma unroll
+    for (int k = 0; k < 9; k++) A.e[k] = c_make<double>(0.5 * D.e[k].im, -0.5 * D.e[k].re);
+    double c[8];
+    c[0] = A.e[1].re + A.e[3].re;
+    c[1] = -A.e[1].im + A.e[3].im;
+    c[2] = A.e[0].re - A.e[4].re;
+    c[3] = A.e[2].re + A.e[6].re;
+    c[4] = -A.e[2].im + A.e[6].im;
+    c[5] = A.e[5].re + A.e[7].re;
+    c[6] = -A.e[5].im + A.e[7].im;
+    c[7] = (A.e[0].re + A.e[4].re - 2.0 * A.e[8].re) * 0.57735026918962573;
+    Obs o;
+    o.e2 = 0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) o.e2 += c[a] * c[a];
+    o.theta = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) o.theta += A.e[k].re * A.e[k].re + A.e[k].im * A.e[k].im;
+    o.f = f;
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_d(mat3<double> &m, const cplx<T> *__restrict__ plane0,
+                                       int64_t stride, int idx) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const cplx<T> v = plane0[k * stride + idx];
+        m.e[k] = c_make<double>((double)v.re, (double)v.im);
+    }
+}
+
+// Deterministic block reduction of three doubles; writes part[3*block + j].
+__device__ __forceinline__ void block_reduce3(Obs o, double *__restrict__ part, unsigned blk) {
+    __shared__ double sh[3][kBlock / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        o.e2 += __shfl_xor(o.e2, off, 64);
+        o.theta += __shfl_xor(o.theta, off, 64);
+        o.f += __shfl_xor(o.f, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wv] = o.e2;
+        sh[1][wv] = o.theta;
+        sh[2][wv] = o.f;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; w++) s += sh[threadIdx.x][w];
+        part[3 * blk + threadIdx.x] = s;
+    }
+}
+
+// Observables of the factored state (u0 of type TU, G of type TG); one thread
+// per site, both parities.
+template <typename TU, typename TG>
+__global__ __launch_bounds__(kBlock) void k_obs_factored(const cplx<TU> *__restrict__ u0,
+                                                         const cplx<TG> *__restrict__ g,
+                                                         int ns, int Vh,
+                                                         double *__restrict__ part) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    Obs o = {0, 0, 0};
+    if (idx < 2 * Vh) {
+        const int p = idx >= Vh, h = idx - p * Vh;
+        const Site s = site_of(h, p, ns);
+        const Nbr nb = neighbours(s, h, ns);
+        const int64_t st = Vh;
+        const cplx<TG> *gq = g + (int64_t)(1 - p) * 9 * st;
+        mat3<double> gn;
+        load_d(gn, g + (int64_t)p * 9 * st, st, h);
+        mat3<double> D;
+        m_zero(D);
+        double f = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            mat3<double> u, gm, tmp, lk;
+            load_d(u, u0 + (int64_t)(p * 3 + i) * 9 * st, st, h);
+            load_d(gm, gq, st, nb.fwd[i]);
+            m_mul(gn, u, tmp);
+            m_mul_nd(tmp, gm, lk);  // U_i(n)
+            accum_link(D, lk, 1.0);
+            f += lk.e[0].re + lk.e[4].re + lk.e[8].re;
+            load_d(u, u0 + (int64_t)((1 - p) * 3 + i) * 9 * st, st, nb.bwd[i]);
+            load_d(gm, gq, st, nb.bwd[i]);
+            m_mul(gm, u, tmp);
+            m_mul_nd(tmp, gn, lk);  // U_i(n-i)
+            accum_link(D, lk, -1.0);
+        }
+        o = site_obs(D, f);
+    }
+    block_reduce3(o, part, blockIdx.x);
+}
+
+// Observables of an AoS FP64 configuration; one thread per site.
+__global__ __launch_bounds__(kBlock) void k_obs_aos(const cplx<double> *__restrict__ cfg,
+                                                    int ns, int64_t V,
+                                                    double *__restrict__ part) {
+    const int64_t lex = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    Obs o = {0, 0, 0};
+    if (lex < V) {
+        const int x = (int)(lex % ns);
+        int64_t r = lex / ns;
+        const int y = (int)(r % ns);
+        r /= ns;
+        const int z = (int)(r % ns);
+        const int t = (int)(r / ns);
+        mat3<double> D;
+        m_zero(D);
+        double f = 0;
+#pragma unroll
+        for (int mu = 1; mu <= 3; mu++) {
+            int c[4] = {t, z, y, x};
+            c[mu] = c[mu] == 0 ? ns - 1 : c[mu] - 1;
+            const int64_t lb = (((int64_t)c[0] * ns + c[1]) * ns + c[2]) * ns + c[3];
+            mat3<double> u;
+            const cplx<double> *pf = cfg + (lex * 4 + mu) * 9;
+#pragma unroll
+            for (int k = 0; k < 9; k++) u.e[k] = pf[k];
+            accum_link(D, u, 1.0);
+            f += u.e[0].re + u.e[4].re + u.e[8].re;
+            const cplx<double> *pb = cfg + (lb * 4 + mu) * 9;
+#pragma unroll
+            for (int k = 0; k < 9; k++) u.e[k] = pb[k];
+            accum_link(D, u, -1.0);
+        }
+        o = site_obs(D, f);
+    }
+    block_reduce3(o, part, blockIdx.x);
+}
+
+// Plaquettes of an AoS FP64 configuration (TrPlaquette, src/measurement.c:38-84):
+// Re Tr[U_mu(n) U_nu(n+mu) U_mu(n+nu)^dag U_nu(n)^dag] / 3, spatial planes
+// (XY, XZ, YZ) into slot 0 and temporal planes (TZ, TY, TX) into slot 1.
+__global__ __launch_bounds__(kBlock) void k_plaquette_aos(const cplx<double> *__restrict__ cfg,
+                                                          int ns, int nt, int64_t V,
+                                                          double *__restrict__ part) {
+    const int64_t lex = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    Obs o = {0, 0, 0};
+    if (lex < V) {
+        const int x = (int)(lex % ns);
+        int64_t r = lex / ns;
+        const int y = (int)(r % ns);
+        r /= ns;
+        const int z = (int)(r % ns);
+        const int t = (int)(r / ns);
+        const int planes[6][2] = {{3, 2}, {3, 1}, {2, 1}, {0, 1}, {0, 2}, {0, 3}};
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int mu = planes[q][0], nu = planes[q][1];
+            int cm[4] = {t, z, y, x}, cn[4] = {t, z, y, x};
+            cm[mu] += 1;
+            if (cm[mu] == (mu == 0 ? nt : ns)) cm[mu] = 0;
+            cn[nu] += 1;
+            if (cn[nu] == (nu == 0 ? nt : ns)) cn[nu] = 0;
+            const int64_t lm = (((int64_t)cm[0] * ns + cm[1]) * ns + cm[2]) * ns + cm[3];
+            const int64_t ln = (((int64_t)cn[0] * ns + cn[1]) * ns + cn[2]) * ns + cn[3];
+            mat3<double> a, b, c, d, ab, abc;
+            const cplx<double> *pa = cfg + (lex * 4 + mu) * 9, *pb = cfg + (lm * 4 + nu) * 9;
+            const cplx<double> *pc = cfg + (ln * 4 + mu) * 9, *pd = cfg + (lex * 4 + nu) * 9;
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                a.e[k] = pa[k];
+                b.e[k] = pb[k];
+                c.e[k] = pc[k];
+                d.e[k] = pd[k];
+            }
+            m_mul(a, b, ab);
+            m_mul_nd(ab, c, abc);
+            // Re Tr(abc d^dag) = sum_ij Re(abc_ij conj(d_ij))
+            double tr = 0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) tr += abc.e[k].re * d.e[k].re + abc.e[k].im * d.e[k].im;
+            if (q < 3)
+                o.e2 += tr / 3.0;
+            else
+                o.theta += tr / 3.0;
+        }
+    }
+    block_reduce3(o, part, blockIdx.x);
+}
+
+// Sums part[3*b + j] over b in fixed order with one block; out[j].
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restrict__ part,
+                                                         int nblocks, double *__restrict__ out) {
+    Obs o = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+        o.e2 += part[3 * b];
+        o.theta += part[3 * b + 1];
+        o.f += part[3 * b + 2];
+    }
+    block_reduce3(o, out, 0);
+}
+
+}  // namespace gfc

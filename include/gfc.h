@@ -1,0 +1,213 @@
+/*
+ * gfc.h -- C ABI of the MI355X Coulomb gauge-fixing engine (libgfc.so).
+ *
+ * Plain C types only: pointers, sizes, ints and doubles.  No exceptions or
+ * device types cross this boundary; every call returns an int status
+ * (0 on success, a negative GFC_E* code otherwise) unless stated, and
+ * gfc_last_error() describes the most recent failure of the calling thread.
+ *
+ * Host-side field layout is the reference's memory/file image
+ * (include/fields.h:33-43 of jesuel-marques/GaugeFix-Coulomb):
+ *   site = ((t*ns + z)*ns + y)*ns + x,  link = site*4 + mu,
+ *   mu: 0 = T, 1 = Z, 2 = Y, 3 = X  (settings.h:57-60),
+ *   one matrix = 9 complex numbers, row-major, (re, im) interleaved:
+ *   18 doubles (144 B) in FP64, 18 floats (72 B) in FP32.
+ */
+#ifndef GFC_H
+#define GFC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GFC_OK 0
+#define GFC_ENOTCONVERGED (-1) /* max sweeps reached or residue stalled
+                                  (gauge_fixing.c:797,811 return -1)       */
+#define GFC_EPARAMS (-2)       /* invalid parameters (gauge_fixing.c:749-751) */
+#define GFC_EDEVICE (-3)       /* HIP runtime error or no usable GPU          */
+#define GFC_EGEOMETRY (-4)     /* ns, nt not positive and even                */
+#define GFC_ENOMEM (-5)        /* host or device allocation failed            */
+#define GFC_EINVAL (-6)        /* bad argument (NULL pointer, bad enum, size) */
+#define GFC_EPROJECT (-7)      /* SU(3) projection met a null row            */
+
+/* ---- parameters -------------------------------------------------------- */
+#define GFC_PROXY_E2 0
+#define GFC_PROXY_THETA 1
+
+/* Mirrors ORGaugeFixingParameters (include/gauge_fixing.h:31-54).  omega_OR
+ * stays a float as in the reference, so 1.95 acts as 1.9500000476837158. */
+typedef struct {
+    double tolerance;
+    int gfix_proxy;                          /* GFC_PROXY_E2 or GFC_PROXY_THETA */
+    unsigned max_sweeps_to_fix;
+    unsigned estimate_sweeps_to_gf_progress;
+    unsigned sweeps_to_reunitarization;
+    int error;                               /* set when the values are invalid */
+    float omega_OR;
+    unsigned hits;
+} gfc_params;
+
+/* Replaces initParametersORDefault (src/gauge_fixing.c:72-84). */
+void gfc_params_default(gfc_params *p);
+/* Replaces initORGaugeFixingParameters (src/gauge_fixing.c:144-244): reads
+ * "key: value" lines (whitespace removed, split at the first ':'); unknown
+ * keys are reported on stderr; an unreadable file keeps the defaults; the
+ * `error` field is set when the result is invalid.  path " " means "no file".
+ * Returns 0, or GFC_EPARAMS when `error` was set. */
+int gfc_params_from_file(const char *path, gfc_params *p);
+/* Replaces validORGaugeFixingParametersQ (src/gauge_fixing.c:45-70): 1 valid. */
+int gfc_params_valid(const gfc_params *p);
+/* Replaces printORGaugeFixingParameters (src/gauge_fixing.c:86-104). */
+void gfc_params_print(const gfc_params *p);
+/* Replaces whenNextConvergenceCheckQ (src/gauge_fixing.c:248-285). */
+int gfc_next_check(unsigned sweep, double residue, const gfc_params *p);
+
+/* ---- geometry ---------------------------------------------------------- */
+typedef struct {
+    int ns; /* spatial extent (x, y, z) */
+    int nt; /* temporal extent */
+} gfc_geom;
+
+/* Replaces initGeometry/validGeometricParametersQ (src/geometry.c:148-228);
+ * additionally requires even ns and nt (red/black independence across the
+ * periodic wrap).  Returns 0 or GFC_EGEOMETRY. */
+int gfc_geom_check(const gfc_geom *g);
+/* Number of sites ns^3 * nt. */
+int64_t gfc_geom_volume(const gfc_geom *g);
+
+/* ---- host I/O (src/fields_io.c) ----------------------------------------- */
+/* Replaces loadConfig (src/fields_io.c:216-305) and, for in_precision 4,
+ * convertCfg_in_work (src/convert_precision_io.c:176-215).  Reads exactly
+ * 4*V matrices of in_precision (4 or 8) bytes per real, byte-swapping each
+ * real when byte_swap != 0, into U (4*V*18 doubles).  Returns 0, -1 (open),
+ * -2 (short file), -3 (trailing bytes), -4 (bad precision) like the
+ * reference. */
+int gfc_load_config(const char *path, const gfc_geom *g, int in_precision, int byte_swap,
+                    double *U);
+/* Writes U (4*V*18 doubles) as precision 4 or 8 with optional byte swap
+ * (writeConfig, src/fields_io.c:405-489). */
+int gfc_write_config(const char *path, const gfc_geom *g, int out_precision, int byte_swap,
+                     const double *U);
+/* Replaces writeGaugeTransf (src/fields_io.c:493-565): raw native-endian
+ * FP64 image of G (V*18 doubles).  Returns 0, -1 (open), -2 (short write). */
+int gfc_write_gauge_transf(const char *path, const gfc_geom *g, const double *G);
+/* Reads a G file written by gfc_write_gauge_transf (loadGaugeTransf,
+ * src/fields_io.c:309-401). */
+int gfc_load_gauge_transf(const char *path, const gfc_geom *g, double *G);
+/* Replaces writeSweepsToGaugefix (gauge_fix_coulomb.c:41-65): appends
+ * "<identifier>\t<sweeps>\n" to sweeps_to_gaugefix_<ns>x<nt>.txt in dir
+ * (NULL = current directory). */
+int gfc_write_sweeps_record(const char *dir, const gfc_geom *g, const char *identifier,
+                            int sweeps);
+
+/* ---- synthetic input (no counterpart in the reference) ------------------ */
+/* Writes a "smooth" configuration U_mu(n) = g(n) exp(i eps H_mu(n)) g(n+mu)^dagger
+ * into U (4*V*18 doubles): g Haar-random SU(3), H traceless Hermitian with unit
+ * variance Gaussian entries, from a counter-based RNG keyed by seed, so the
+ * result does not depend on the thread count.  Returns 0 or GFC_EGEOMETRY. */
+int gfc_generate_smooth(const gfc_geom *g, uint64_t seed, double eps, double *U);
+/* Rounds a FP64 field to FP32 (n matrices). */
+void gfc_field_to_fp32(const double *in, float *out, int64_t n_matrices);
+
+/* ---- device engine ------------------------------------------------------ */
+#define GFC_PREC_FP64 0  /* every sweep in FP64 (drop-in)                      */
+#define GFC_PREC_MIXED 1 /* FP32 sweeps until the residue reaches
+                            mixed_switch, then FP64 sweeps to the tolerance   */
+
+typedef struct {
+    int precision;          /* GFC_PREC_FP64 (default) or GFC_PREC_MIXED      */
+    double mixed_switch;    /* MIXED: FP32 -> FP64 hand-off residue (1e-9)     */
+    int mixed_probe_every;  /* MIXED: half-sweeps between FP32-phase residue
+                               probes (default 50); scheduled checks probe too */
+    int verbose;            /* 1: print "Sweeps: ..." lines like the reference */
+    int time_sweeps;        /* 1: bracket every sweep kernel with HIP events on
+                               the job's stream and report their total in
+                               gfc_report.seconds_sweeps                       */
+} gfc_options;
+
+typedef struct {
+    int sweeps;             /* half-sweeps performed                          */
+    int sweeps_fp32;        /* of which in FP32 (MIXED)                        */
+    int residue_evals;      /* residue evaluations (checks + probes)          */
+    double residue_initial; /* proxy before the first sweep                    */
+    double residue_final;   /* proxy at the last evaluation                    */
+    double seconds;         /* wall time of the fixing loop on the device      */
+    double seconds_sweeps;  /* of which in sweep kernels (HIP events)          */
+} gfc_report;
+
+void gfc_options_default(gfc_options *o);
+
+typedef struct gfc_ctx gfc_ctx; /* one HIP device + stream */
+typedef struct gfc_job gfc_job; /* one lattice resident in HBM */
+
+const char *gfc_last_error(void);
+/* Library version string. */
+const char *gfc_version(void);
+
+int gfc_ctx_create(int device, gfc_ctx **ctx);
+void gfc_ctx_destroy(gfc_ctx *ctx);
+/* Bytes of device memory a job of this geometry needs for the given
+ * precision (used for admission by footprint). */
+int64_t gfc_job_footprint(const gfc_geom *g, int precision);
+
+/* Allocates the device fields of one lattice. */
+int gfc_job_create(gfc_ctx *ctx, const gfc_geom *g, int precision, gfc_job **job);
+void gfc_job_destroy(gfc_job *job);
+/* Copies a host configuration (4*V matrices; precision 8 = double, 4 = float)
+ * to the device; G is set to the identity. */
+int gfc_job_upload_config(gfc_job *job, const void *U, int in_precision);
+/* Projects all 4*V links onto SU(3) (main's initial reunitarizeField,
+ * gauge_fix_coulomb.c:126-131).  Returns GFC_EPROJECT if a link had a null
+ * row, like reunitarizeField's -1. */
+int gfc_job_reunitarize(gfc_job *job);
+/* e2 (calculate_e2, src/gauge_fixing.c:389-442), theta (calculateTheta,
+ * :341-384) and F (calculateF, :289-337) of the job's current U. */
+int gfc_job_observables(gfc_job *job, double *e2, double *theta, double *F);
+/* averagePlaquette(U, "spatial"/"temporal") (src/measurement.c:108-155). */
+int gfc_job_plaquette(gfc_job *job, double *spatial, double *temporal);
+/* Replaces gaugefixOverrelaxation (src/gauge_fixing.c:713-826, declared at
+ * include/gauge_fixing.h:67-69) on the resident job: G <- 1, red/black
+ * overrelaxation sweeps with the reference's check schedule, stall test,
+ * max-sweeps test and reunitarization every sweeps_to_reunitarization
+ * sweeps; U and G are left gauge-fixed on the device.  Returns the sweep
+ * count, GFC_ENOTCONVERGED or GFC_EPARAMS (or a device error). */
+int gfc_job_fix(gfc_job *job, const gfc_params *p, const gfc_options *o, gfc_report *rep);
+/* gfc_job_fix that also stores the proxy measured after each half-sweep k
+ * (before that sweep's reunitarization) in trace[k-1], for k <= trace_len.
+ * Costs one residue evaluation per sweep; meant for convergence studies. */
+int gfc_job_fix_trace(gfc_job *job, const gfc_params *p, const gfc_options *o,
+                      gfc_report *rep, double *trace, int trace_len);
+/* Copies the job's current U (4*V*18 doubles) / G (V*18 doubles) to the host. */
+int gfc_job_download_config(gfc_job *job, double *U);
+int gfc_job_download_gauge(gfc_job *job, double *G);
+/* Keeps a device-side copy of the current U (allocated on first use) /
+ * restores it and resets G to the identity: re-fixing the same input without
+ * a host transfer. */
+int gfc_job_save_config(gfc_job *job);
+int gfc_job_restore_config(gfc_job *job);
+/* Copies one link U_mu(site) (18 doubles) of the job's current U to the host
+ * (what gauge_fix_coulomb.c:120,132 prints with printMatrix3x3). */
+int gfc_job_get_link(gfc_job *job, int64_t site, int mu, double *out18);
+
+/* The drop-in entry point: gaugefixOverrelaxation(U, G, params) on host
+ * arrays (include/gauge_fixing.h:67-69).  U (4*V*18 doubles) and G (V*18
+ * doubles) are updated in place with the same return codes.  o and rep may
+ * be NULL. */
+int gfc_fix_coulomb_or(gfc_ctx *ctx, const gfc_geom *g, const gfc_params *p,
+                       const gfc_options *o, double *U, double *G, gfc_report *rep);
+/* reunitarizeField (src/fields.c:144-187) on a host array of n matrices. */
+int gfc_reunitarize_field(gfc_ctx *ctx, double *field, int64_t n);
+/* calculate_e2 / calculateTheta / calculateF / averagePlaquette of a host
+ * configuration (4*V*18 doubles), computed on the device. */
+int gfc_config_observables(gfc_ctx *ctx, const gfc_geom *g, const double *U, double *e2,
+                           double *theta, double *F, double *plaq_spatial,
+                           double *plaq_temporal);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GFC_H */

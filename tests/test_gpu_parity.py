@@ -1,0 +1,233 @@
+"""Parity of the HIP engine (through the C ABI) with the compiled reference.
+
+Expected values are the reference's own outputs (tests/golden/*.npz written by
+oracle/ref_golden.c) or, where stated, the CPU oracle run here on the same
+seeded input.  Tolerances are those of the FP64 drop-in contract:
+  per-sweep e2 within 1e-12 relative, G element-wise within 1e-10 after a
+  fixed sweep count, average plaquette unchanged to 1e-14, sweep count to
+  convergence within +-1% (exact here).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import gfc
+import oracle as orc
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+E2_REL = 1e-12
+G_ABS = 1e-10
+PLAQ_ABS = 1e-14
+
+
+def _config(f):
+    geom = gfc.Geometry(int(f["ns"]), int(f["nt"]))
+    U = gfc.generate_smooth(geom, int(f["seed"]), float(f["eps"]))
+    assert hashlib.sha256(U.tobytes()).hexdigest() == str(f["config_sha256"])
+    return geom, U
+
+
+def _job(ctx, geom, U, precision=gfc.PREC_FP64):
+    job = gfc.Job(ctx, geom, precision)
+    job.upload_config(U)
+    assert job.reunitarize() == 0
+    return job
+
+
+def _fixed_k(k):
+    return gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=100000000,
+                      max_sweeps_to_fix=k, sweeps_to_reunitarization=10)
+
+
+def test_reunitarize_and_observables_4x8(ctx):
+    f = golden("lat_4x8.npz")
+    geom, U = _config(f)
+    job = gfc.Job(ctx, geom)
+    job.upload_config(U)
+    o = np.array([*job.observables(), *job.plaquette()])
+    assert np.allclose(o, f["obs_raw"], rtol=1e-13, atol=1e-15)
+    assert job.reunitarize() == 0
+    Ur = job.download_config()
+    assert np.max(np.abs(Ur - f["U_reunit"])) < 1e-14
+    o = np.array([*job.observables(), *job.plaquette()])
+    assert np.allclose(o, f["obs_reunit"], rtol=1e-13, atol=1e-15)
+    job.close()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 10, 11, 200])
+def test_state_after_k_sweeps_4x8(ctx, k):
+    f = golden("lat_4x8.npz")
+    geom, U = _config(f)
+    job = _job(ctx, geom, U)
+    rc, rep = job.fix(_fixed_k(k))
+    assert rc == gfc.ENOTCONVERGED and rep.sweeps == k
+    G = job.download_gauge()
+    dG = np.max(np.abs(G - f[f"G_{k}"]))
+    print(f"k={k} max|dG|={dG:.3e}")
+    assert dG < G_ABS
+    i = list(f["traj_k"]).index(k)
+    e2, th, F = job.observables()
+    ps, pt = job.plaquette()
+    ref = f["traj_obs"][i]
+    assert abs(e2 - ref[0]) / ref[0] < E2_REL
+    assert abs(th - ref[1]) / ref[1] < E2_REL
+    assert abs(F - ref[2]) < 1e-13
+    assert abs(ps - ref[3]) < PLAQ_ABS and abs(pt - ref[4]) < PLAQ_ABS
+    if k == 200:
+        # U (including the temporal links, rebuilt as G U0 G^dagger)
+        assert np.max(np.abs(job.download_config() - f["U_200"])) < 1e-12
+    job.close()
+
+
+@pytest.mark.parametrize("name", ["lat_4x8.npz", "lat_8x16.npz"])
+def test_e2_every_sweep(ctx, name):
+    f = golden(name)
+    geom, U = _config(f)
+    job = _job(ctx, geom, U)
+    rc, rep, tr = job.fix(_fixed_k(200), trace_len=200)
+    ref = f["traj_obs"][:, 0]
+    rel = np.abs(tr - ref) / ref
+    print(f"{name}: max rel e2 diff over 200 sweeps {rel.max():.3e}")
+    assert rel.max() < E2_REL
+    G = job.download_gauge()
+    assert np.max(np.abs(G[f["G_sites"]] - f["G_200"])) < G_ABS
+    job.close()
+
+
+@pytest.mark.parametrize("name,est", [("lat_4x8.npz", 1000), ("lat_4x8.npz", 0),
+                                      ("lat_8x16.npz", 1000), ("lat_8x16.npz", 0)])
+def test_full_run_to_tolerance(ctx, name, est):
+    f = golden(name)
+    geom, U = _config(f)
+    job = _job(ctx, geom, U)
+    p0 = job.plaquette()
+    p = gfc.Params(tolerance=1e-16, estimate_sweeps_to_gf_progress=est,
+                   sweeps_to_reunitarization=10)
+    rc, rep = job.fix(p)
+    assert rc == int(f[f"full_est{est}_rc"]), (rc, int(f[f"full_est{est}_rc"]))
+    e2, th, F = job.observables()
+    ref = f[f"full_est{est}_obs"]
+    assert e2 <= 1e-16
+    assert abs(F - ref[2]) < 1e-12
+    p1 = job.plaquette()
+    assert abs(p1[0] - p0[0]) < PLAQ_ABS and abs(p1[1] - p0[1]) < PLAQ_ABS
+    if est == 1000:
+        G = job.download_gauge()
+        Gref = f["full_est1000_G"]
+        sel = f["G_sites"] if len(Gref) < geom.volume else slice(None)
+        assert np.max(np.abs(G[sel] - Gref)) < G_ABS
+    job.close()
+
+
+def test_drop_in_host_arrays(ctx):
+    """gfc_fix_coulomb_or mutates host U and G in place like gaugefixOverrelaxation."""
+    f = golden("lat_4x8.npz")
+    geom, U = _config(f)
+    U = np.ascontiguousarray(f["U_reunit"])
+    G = np.zeros((geom.volume, 3, 3), np.complex128)
+    rc, rep = gfc.fix_coulomb_or(ctx, geom, _fixed_k(200), U, G)
+    assert rc == -1
+    assert np.max(np.abs(G - f["G_200"])) < G_ABS
+    assert np.max(np.abs(U - f["U_200"])) < 1e-12
+
+
+def test_config2_32x64(ctx):
+    """Config 2: 32^3x64, eps 0.3 -- e2 after k sweeps, G after 200, plaquette
+    invariance and the sweep count to 1e-16 against the reference."""
+    f = golden("lat_32x64.npz")
+    geom, U = _config(f)
+    job = _job(ctx, geom, U)
+    p0 = job.plaquette()
+    assert np.allclose(p0, f["obs_reunit"][3:], atol=1e-14)
+    for i, k in enumerate(f["traj_k"]):
+        rc, rep = job.fix(_fixed_k(int(k)))
+        e2, th, F = job.observables()
+        ref = f["traj_obs"][i]
+        print(f"k={k} e2={e2:.6e} ref={ref[0]:.6e} rel={abs(e2 - ref[0]) / ref[0]:.2e}")
+        assert abs(e2 - ref[0]) / ref[0] < E2_REL
+        p1 = job.plaquette()
+        assert abs(p1[0] - p0[0]) < PLAQ_ABS and abs(p1[1] - p0[1]) < PLAQ_ABS
+        if k == 200:
+            G = job.download_gauge()
+            dG = np.max(np.abs(G[f["G_sites"]] - f["G_200"]))
+            print(f"G after 200: max|dG| = {dG:.3e}")
+            assert dG < G_ABS
+        job.upload_config(U)
+        job.reunitarize()
+    rc, rep = job.fix(gfc.Params(sweeps_to_reunitarization=10))
+    ref_rc = int(f["full_est1000_rc"])
+    print(f"sweeps {rc} reference {ref_rc}")
+    assert abs(rc - ref_rc) <= 0.01 * ref_rc
+    assert job.observables()[0] <= 1e-16
+    job.close()
+
+
+def test_stall_and_max_sweeps(ctx):
+    """hits = 0 with omega = 1 leaves every site unchanged, so the residue
+    stalls at the first check (gauge_fixing.c:794-797 -> -1); a small
+    max_sweeps_to_fix returns -1 too (:810-812)."""
+    f = golden("lat_4x8.npz")
+    geom, U = _config(f)
+    job = _job(ctx, geom, U)
+    rc, rep = job.fix(gfc.Params(hits=0, omega_OR=1.0, estimate_sweeps_to_gf_progress=0))
+    assert rc == -1 and rep.sweeps == 10
+    G = job.download_gauge()
+    assert np.max(np.abs(G - np.eye(3))) < 1e-15
+    job.upload_config(U)
+    job.reunitarize()
+    rc, rep = job.fix(gfc.Params(max_sweeps_to_fix=7))
+    assert rc == -1 and rep.sweeps == 7
+    rc, rep = job.fix(gfc.Params(omega_OR=2.0))
+    assert rc == gfc.EPARAMS
+    job.close()
+
+
+def test_null_link_reunitarize_fails(ctx):
+    geom = gfc.Geometry(4, 4)
+    U = gfc.generate_smooth(geom, 5, 0.2)
+    U[3, 2] = 0
+    job = gfc.Job(ctx, geom)
+    job.upload_config(U)
+    with pytest.raises(gfc.GfcError) as e:
+        job.reunitarize()
+    assert e.value.code == gfc.EPROJECT
+    job.close()
+
+
+def test_theta_proxy_matches_oracle(ctx):
+    geom = gfc.Geometry(4, 8)
+    U = gfc.generate_smooth(geom, 11, 0.3)
+    job = _job(ctx, geom, U)
+    p = gfc.Params(gfix_proxy=gfc.PROXY_THETA, tolerance=1e-17, sweeps_to_reunitarization=10)
+    rc, rep = job.fix(p)
+    U1 = U.copy()
+    og = orc.geom(4, 8)
+    orc.reunitarize(U1)
+    G = np.zeros((geom.volume, 3, 3), np.complex128)
+    rc_o, _ = orc.gaugefix(U1, G, og, orc.params(tolerance=1e-17, proxy=1, reunit_every=10))
+    assert rc == rc_o
+    assert np.max(np.abs(job.download_gauge() - G)) < G_ABS
+    job.close()
+
+
+def test_mixed_precision_reaches_tolerance(ctx):
+    """FP32 sweeps to e2 <= 1e-9, then FP64: e2 <= 1e-16, functional equal to
+    the FP64 result to 1e-10 relative, plaquette unchanged."""
+    f = golden("lat_8x16.npz")
+    geom, U = _config(f)
+    job = _job(ctx, geom, U, gfc.PREC_MIXED)
+    p0 = job.plaquette()
+    p = gfc.Params(sweeps_to_reunitarization=10)
+    rc, rep = job.fix(p, gfc.Options(precision=gfc.PREC_MIXED))
+    assert rc > 0 and rep.sweeps_fp32 > 0
+    e2, th, F = job.observables()
+    print(f"mixed: sweeps {rc} (fp32 {rep.sweeps_fp32}) e2 {e2:.3e} F {F:.15f}")
+    assert e2 <= 1e-16
+    Fref = f["full_est1000_obs"][2]
+    assert abs(F - Fref) / abs(Fref) < 1e-10
+    p1 = job.plaquette()
+    assert abs(p1[0] - p0[0]) < PLAQ_ABS and abs(p1[1] - p0[1]) < PLAQ_ABS
+    job.close()

@@ -1,0 +1,149 @@
+"""The CPU oracle (oracle/gfc_oracle.c) against the compiled reference.
+
+Every expected value here was written by the reference's own code
+(oracle/ref_golden.c linked with objects built from the reference sources;
+see tests/golden/make_golden.py).  The oracle follows the reference's
+operation order, so most comparisons are at the last-bit level.
+"""
+import numpy as np
+import pytest
+
+import gfc
+import oracle as orc
+from conftest import golden
+
+REL = 1e-14
+
+
+def _rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300)
+
+
+def test_prims_det_inverse_prod():
+    f = golden("prims.npz")
+    A = f["A"]
+    for i in range(len(A)):
+        assert abs(orc.det(A[i]) - f["det"][i]) <= 1e-15 * max(1, abs(f["det"][i]))
+        assert np.max(np.abs(orc.inverse(A[i]) - f["inverse"][i])) <= 1e-13 * np.max(
+            np.abs(f["inverse"][i]))
+        # the harness multiplies by the next raw draw; every 8th stored input
+        # (i % 8 == 7) is that draw projected and perturbed, so skip those pairs
+        B = A[i + 1] if i + 1 < len(A) and (i + 1) % 8 != 7 else None
+        if B is not None:
+            assert np.max(np.abs(orc.prod(A[i], B) - f["prod_next"][i])) < 1e-15
+
+
+def test_prims_project_power_cm_decompose():
+    f = golden("prims.npz")
+    A = f["A"]
+    for i in range(len(A)):
+        st, P = orc.project_su3(A[i])
+        assert st == f["proj_status"][i]
+        assert np.max(np.abs(P - f["proj"][i])) < 1e-15
+        st, P = orc.power_binomial(A[i], float(np.float32(1.95)))
+        assert st == f["pow195_status"][i]
+        assert np.max(np.abs(P - f["pow195"][i])) < 1e-15
+        st, P = orc.power_binomial(A[i], 1.0)
+        assert st == f["pow1_status"][i]
+        assert np.max(np.abs(P - f["pow1"][i])) < 1e-15
+        for sub, name in enumerate("RST"):
+            assert np.max(np.abs(orc.cm_step(A[i], sub) - f[f"cm_{name}"][i])) < 1e-15
+        assert np.max(np.abs(orc.decompose(A[i]) - f["decomp"][i])) < 1e-15
+
+
+def _config(f):
+    geom = gfc.Geometry(int(f["ns"]), int(f["nt"]))
+    U = gfc.generate_smooth(geom, int(f["seed"]), float(f["eps"]))
+    import hashlib
+    assert hashlib.sha256(U.tobytes()).hexdigest() == str(f["config_sha256"]), \
+        "generator output changed"
+    return geom, U
+
+
+def _obs(U, g):
+    return np.array([orc.e2(U, g), orc.theta(U, g), orc.functional(U, g),
+                     orc.plaquette(U, g, 0), orc.plaquette(U, g, 1)])
+
+
+@pytest.mark.parametrize("name", ["lat_4x8.npz", "lat_8x16.npz"])
+def test_observables_and_reunitarization(name):
+    f = golden(name)
+    geom, U = _config(f)
+    g = orc.geom(geom.ns, geom.nt)
+    assert np.allclose(_obs(U, g), f["obs_raw"], rtol=REL, atol=1e-15)
+    assert orc.reunitarize(U) == f["reunit_status"]
+    if "U_reunit" in f:
+        assert np.max(np.abs(U - f["U_reunit"])) < 1e-15
+    assert np.allclose(_obs(U, g), f["obs_reunit"], rtol=REL, atol=1e-15)
+
+
+def test_trajectory_4x8_all_sweeps():
+    """e2 after every half-sweep k = 1..200 (reference: one run per k)."""
+    f = golden("lat_4x8.npz")
+    geom, U = _config(f)
+    g = orc.geom(geom.ns, geom.nt)
+    orc.reunitarize(U)
+    G = np.zeros((geom.volume, 3, 3), np.complex128)
+    p = orc.params(tolerance=1e-300, estimate=100000000, max_sweeps=200, reunit_every=10)
+    rc, tr = orc.gaugefix(U, G, g, p, trace_len=200)
+    assert rc == -1
+    ref = f["traj_obs"][:, 0]
+    assert np.max(np.abs(tr - ref) / ref) < 1e-13
+    # the state after 200 sweeps (returned before that sweep's reunitarization)
+    assert np.max(np.abs(G - f["G_200"])) < 1e-13
+    assert np.max(np.abs(U - f["U_200"])) < 1e-13
+    assert np.allclose(_obs(U, g), f["traj_obs"][-1], rtol=1e-12)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 10, 11])
+def test_state_after_k_sweeps_4x8(k):
+    f = golden("lat_4x8.npz")
+    geom, U = _config(f)
+    g = orc.geom(geom.ns, geom.nt)
+    orc.reunitarize(U)
+    G = np.zeros((geom.volume, 3, 3), np.complex128)
+    p = orc.params(tolerance=1e-300, estimate=100000000, max_sweeps=k, reunit_every=10)
+    rc, _ = orc.gaugefix(U, G, g, p)
+    assert rc == -1
+    assert np.max(np.abs(G - f[f"G_{k}"])) < 1e-14
+    i = list(f["traj_k"]).index(k)
+    assert np.allclose(_obs(U, g), f["traj_obs"][i], rtol=1e-13)
+
+
+@pytest.mark.parametrize("est", [1000, 0])
+def test_full_run_sweep_count_4x8(est):
+    f = golden("lat_4x8.npz")
+    geom, U = _config(f)
+    g = orc.geom(geom.ns, geom.nt)
+    orc.reunitarize(U)
+    G = np.zeros((geom.volume, 3, 3), np.complex128)
+    p = orc.params(tolerance=1e-16, estimate=est, reunit_every=10)
+    rc, _ = orc.gaugefix(U, G, g, p)
+    assert rc == f[f"full_est{est}_rc"]
+    o = _obs(U, g)
+    ref = f[f"full_est{est}_obs"]
+    assert np.allclose(o[2:], ref[2:], rtol=1e-13)
+    assert o[0] <= 1e-16 and ref[0] <= 1e-16
+    if est == 1000:
+        assert np.max(np.abs(G - f["full_est1000_G"])) < 1e-12
+
+
+@pytest.mark.slow
+def test_trajectory_8x16_config1():
+    """Config 1 (8^3x16): e2 after each of 200 half-sweeps, G at 200, and the
+    sweep counts to tolerance under the default and the dense schedule."""
+    f = golden("lat_8x16.npz")
+    geom, U0 = _config(f)
+    g = orc.geom(geom.ns, geom.nt)
+    orc.reunitarize(U0)
+    U = U0.copy()
+    G = np.zeros((geom.volume, 3, 3), np.complex128)
+    p = orc.params(tolerance=1e-300, estimate=100000000, max_sweeps=200, reunit_every=10)
+    rc, tr = orc.gaugefix(U, G, g, p, trace_len=200)
+    ref = f["traj_obs"][:, 0]
+    assert np.max(np.abs(tr - ref) / ref) < 1e-13
+    assert np.max(np.abs(G[f["G_sites"]] - f["G_200"])) < 1e-13
+    for est in (1000, 0):
+        U = U0.copy()
+        rc, _ = orc.gaugefix(U, G, g, orc.params(estimate=est, reunit_every=10))
+        assert rc == f[f"full_est{est}_rc"]
