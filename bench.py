@@ -43,12 +43,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(kernel_tag: str):
-    """HBM bytes per launch from a committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.txt")
+def pmc_traffic(ns: int, nt: int, tag: str):
+    """HBM bytes per launch of a sweep kernel from the committed rocprofv3 PMC
+    summary for this lattice (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"sweep_{ns}x{nt}_pmc.txt")
     if not os.path.exists(path):
         return None
-    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(tag + r"_hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -109,6 +110,23 @@ def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
                       f"warm-up call; time of the (1+{n})-sweep call minus the 1-sweep call"}
 
 
+def profile_kernels(ctx, job, n: int):
+    """n FP64 half-sweeps, n FP32 half-sweeps and 1 GiB copies, for counters."""
+    import gfc
+    fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=100000000,
+                       max_sweeps_to_fix=n, sweeps_to_reunitarization=10 ** 6)
+    job.restore_config()
+    rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_FP64))
+    log(f"profile: {rep.sweeps} FP64 half-sweeps")
+    job.restore_config()
+    rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0,
+                                         mixed_probe_every=10 ** 6))
+    log(f"profile: {rep.sweeps_fp32} FP32 half-sweeps")
+    sec = ctx.copy_probe(2 ** 30, reps=3)
+    log(f"profile: copied 1 GiB x 4 with k_copy16 (calibration kernel), "
+        f"{2 * 2 ** 30 / sec / 1e9:.1f} GB/s read+write")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +139,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--profile-kernels", type=int, default=0, metavar="N",
+                    help="only run N FP64 and N FP32 half-sweeps plus a 1 GiB device copy "
+                         "(calibration for rocprofv3 PMC counters), then exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine runs one lattice per GPU and the "
@@ -144,6 +165,12 @@ def main():
     job.upload_config(U32)
     job.reunitarize()
     job.save_config()
+    copy_s = ctx.copy_probe(2 ** 30, reps=5)
+    copy_gbs = 2 * 2 ** 30 / copy_s / 1e9
+    log(f"device copy: {copy_gbs:.1f} GB/s (read + write)")
+    if a.profile_kernels:
+        profile_kernels(ctx, job, a.profile_kernels)
+        return
     plaq0 = job.plaquette()
     fp64 = gfc.Options(precision=gfc.PREC_FP64)
     mixed = gfc.Options(precision=gfc.PREC_MIXED)
@@ -237,7 +264,9 @@ def main():
         **out_mixed,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic(f"sweep_fp64_{a.ns}x{a.nt}")},
+                     "traffic": pmc_traffic(a.ns, a.nt, "fp64")},
+        "copy_kernel_GBs": round(copy_gbs, 1),
+        "sweep_frac_of_copy": round(achieved / copy_gbs, 4),
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

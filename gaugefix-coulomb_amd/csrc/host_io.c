@@ -172,5 +172,6 @@ int gfc_write_sweeps_record(const char *dir, const gfc_geom *g, const char *iden
 
 void gfc_field_to_fp32(const double *in, float *out, int64_t n_matrices) {
     const int64_t n = n_matrices * 18;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) out[i] = (float)in[i];
 }

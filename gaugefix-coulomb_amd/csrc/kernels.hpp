@@ -480,6 +480,13 @@ __global__ __launch_bounds__(kBlock) void k_plaquette_aos(const cplx<double> *__
     block_reduce3(o, part, blockIdx.x);
 }
 
+// Streaming copy of n 16-byte elements (bandwidth probe / counter calibration).
+__global__ __launch_bounds__(kBlock) void k_copy16(const double2 *__restrict__ in,
+                                                   double2 *__restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
 // Sums part[3*b + j] over b in fixed order with one block; out[j].
 __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restrict__ part,
                                                          int nblocks, double *__restrict__ out) {

@@ -110,6 +110,7 @@ def _load():
         "gfc_version": (C.c_char_p, []),
         "gfc_ctx_create": (I, [I, C.POINTER(P)]),
         "gfc_ctx_destroy": (None, [P]),
+        "gfc_ctx_copy_probe": (I, [P, I64, I, D]),
         "gfc_job_footprint": (I64, [C.POINTER(_Geom), I]),
         "gfc_job_create": (I, [P, C.POINTER(_Geom), I, C.POINTER(P)]),
         "gfc_job_destroy": (None, [P]),
@@ -144,7 +145,7 @@ EXPORTED = sorted(
      "gfc_write_config", "gfc_write_gauge_transf", "gfc_load_gauge_transf",
      "gfc_write_sweeps_record", "gfc_generate_smooth", "gfc_field_to_fp32",
      "gfc_options_default", "gfc_last_error", "gfc_version", "gfc_ctx_create",
-     "gfc_ctx_destroy", "gfc_job_footprint", "gfc_job_create", "gfc_job_destroy",
+     "gfc_ctx_destroy", "gfc_ctx_copy_probe", "gfc_job_footprint", "gfc_job_create", "gfc_job_destroy",
      "gfc_job_upload_config", "gfc_job_reunitarize", "gfc_job_observables",
      "gfc_job_plaquette", "gfc_job_fix", "gfc_job_fix_trace", "gfc_job_download_config",
      "gfc_job_download_gauge", "gfc_job_get_link", "gfc_job_save_config",
@@ -314,6 +315,12 @@ class Context:
     def __init__(self, device: int = 0):
         self._p = C.c_void_p()
         _check(lib.gfc_ctx_create(device, C.byref(self._p)))
+
+    def copy_probe(self, nbytes: int, reps: int = 5) -> float:
+        """Mean seconds of a device copy of nbytes (streaming 16-byte copy kernel)."""
+        s = C.c_double()
+        _check(lib.gfc_ctx_copy_probe(self._p, nbytes, reps, C.byref(s)))
+        return s.value
 
     def close(self):
         if self._p:

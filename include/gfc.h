@@ -150,6 +150,12 @@ const char *gfc_version(void);
 
 int gfc_ctx_create(int device, gfc_ctx **ctx);
 void gfc_ctx_destroy(gfc_ctx *ctx);
+/* Device copy of `bytes` bytes (a plain 16-byte-per-lane streaming copy
+ * kernel) repeated `reps` times on the context's stream; *seconds receives
+ * the mean kernel time from HIP events.  Measures the achievable HBM copy
+ * bandwidth and calibrates memory-traffic counters (no counterpart in the
+ * reference). */
+int gfc_ctx_copy_probe(gfc_ctx *ctx, int64_t bytes, int reps, double *seconds);
 /* Bytes of device memory a job of this geometry needs for the given
  * precision (used for admission by footprint). */
 int64_t gfc_job_footprint(const gfc_geom *g, int precision);
