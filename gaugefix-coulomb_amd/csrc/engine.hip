@@ -153,10 +153,11 @@ int gfc_ctx_copy_probe(gfc_ctx *c, int64_t bytes, int reps, double *seconds) {
 int64_t gfc_job_footprint(const gfc_geom *g, int precision) {
     if (gfc_geom_check(g)) return -1;
     const int64_t V = gfc_geom_volume(g);
+    // u0 and G keep two rows (6 complex numbers) per SU(3) matrix
     int64_t b = V * 4 * 144      // cfg
-                + V * 3 * 144    // u0
-                + V * 144;       // g
-    if (precision == GFC_PREC_MIXED) b += V * 3 * 72 + V * 72;
+                + V * 3 * 96     // u0
+                + V * 96;        // g
+    if (precision == GFC_PREC_MIXED) b += V * 3 * 48 + V * 48;
     return b;
 }
 
@@ -220,11 +221,12 @@ extern "C" int gfc_job_create(gfc_ctx *ctx, const gfc_geom *g, int precision, gf
     j->Vh = (int)(V / 2);
     j->precision = precision;
     hipError_t e = hipMalloc(&j->cfg, (size_t)V * 4 * 9 * sizeof(cplx<double>));
-    if (e == hipSuccess) e = hipMalloc(&j->u0, (size_t)V * 3 * 9 * sizeof(cplx<double>));
-    if (e == hipSuccess) e = hipMalloc(&j->g, (size_t)V * 9 * sizeof(cplx<double>));
+    const size_t np = gfc::kPlanes;
+    if (e == hipSuccess) e = hipMalloc(&j->u0, (size_t)V * 3 * np * sizeof(cplx<double>));
+    if (e == hipSuccess) e = hipMalloc(&j->g, (size_t)V * np * sizeof(cplx<double>));
     if (e == hipSuccess && precision == GFC_PREC_MIXED) {
-        e = hipMalloc(&j->u0f, (size_t)V * 3 * 9 * sizeof(cplx<float>));
-        if (e == hipSuccess) e = hipMalloc(&j->gf, (size_t)V * 9 * sizeof(cplx<float>));
+        e = hipMalloc(&j->u0f, (size_t)V * 3 * np * sizeof(cplx<float>));
+        if (e == hipSuccess) e = hipMalloc(&j->gf, (size_t)V * np * sizeof(cplx<float>));
     }
     if (e != hipSuccess) {
         gfc_job_destroy(j);
@@ -250,7 +252,8 @@ extern "C" int gfc_job_upload_config(gfc_job *j, const void *U, int in_precision
     if (in_precision == 8) {
         HIP_TRY(hipMemcpyAsync(j->cfg, U, nreal * sizeof(double), hipMemcpyHostToDevice, s));
     } else {
-        // FP32 staging lands in the u0 planes (large enough: 27V complex doubles)
+        // FP32 staging lands in the u0 planes: 18V complex doubles = 288V bytes,
+        // exactly the 4V x 18 floats of an FP32 configuration
         float *stage = reinterpret_cast<float *>(j->u0);
         HIP_TRY(hipMemcpyAsync(stage, U, nreal * sizeof(float), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(gfc::k_f32_to_f64, dim3(nblocks(nreal)), dim3(kBlock), 0, s, stage,
@@ -356,35 +359,45 @@ int materialize(gfc_job *j) {
     return GFC_OK;
 }
 
-int half_sweep(gfc_job *j, int parity, bool fp32, double omega, int hits) {
+// Host-side sweep state: which G array is live.
+struct SweepState {
+    bool fp32 = false;
+};
+
+int half_sweep(gfc_job *j, SweepState &st, int parity, double omega, int hits) {
     hipStream_t s = j->ctx->stream;
     const unsigned b = nblocks(j->Vh);
-    if (fp32)
-        hipLaunchKernelGGL(gfc::k_sweep<float>, dim3(b), dim3(kBlock), 0, s, j->u0f, j->gf,
-                           j->geom.ns, j->Vh, parity, (float)omega, hits, 0, j->Vh);
+    const int ns = j->geom.ns, Vh = j->Vh;
+    if (st.fp32)
+        hipLaunchKernelGGL(gfc::k_sweep<float>, dim3(b), dim3(kBlock), 0, s, j->u0f, j->gf, ns,
+                           Vh, parity, (float)omega, hits, 0, Vh);
     else
-        hipLaunchKernelGGL(gfc::k_sweep<double>, dim3(b), dim3(kBlock), 0, s, j->u0, j->g,
-                           j->geom.ns, j->Vh, parity, omega, hits, 0, j->Vh);
+        hipLaunchKernelGGL(gfc::k_sweep<double>, dim3(b), dim3(kBlock), 0, s, j->u0, j->g, ns,
+                           Vh, parity, omega, hits, 0, Vh);
     LAUNCH_CHECK();
     return GFC_OK;
 }
 
-int reunit_g(gfc_job *j, bool fp32) {
+// Projects G on parities [p0, p0 + np) of the live array.
+int reunit_g(gfc_job *j, bool fp32, int p0, int np) {
     hipStream_t s = j->ctx->stream;
-    const unsigned b = nblocks(2 * (int64_t)j->Vh);
+    const unsigned b = nblocks((int64_t)np * j->Vh);
     if (fp32)
-        hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh);
+        hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh, p0,
+                           np);
     else
-        hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh);
+        hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh, p0,
+                           np);
     LAUNCH_CHECK();
     return GFC_OK;
 }
 
-int promote_g(gfc_job *j) {
+int promote_g(gfc_job *j, SweepState &st) {
     hipStream_t s = j->ctx->stream;
     const unsigned b = nblocks(2 * (int64_t)j->Vh);
     hipLaunchKernelGGL(gfc::k_g_promote, dim3(b), dim3(kBlock), 0, s, j->gf, j->g, j->Vh);
     LAUNCH_CHECK();
+    st.fp32 = false;
     return GFC_OK;
 }
 
@@ -453,7 +466,8 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     const double omega = (double)p->omega_OR;  // float in the reference
     const int hits = (int)p->hits;
     const int which = p->gfix_proxy == GFC_PROXY_THETA ? 1 : 0;
-    bool phase32 = o.precision == GFC_PREC_MIXED;
+    SweepState st;
+    st.fp32 = o.precision == GFC_PREC_MIXED;
     double ob[3];
 
     SweepTimer timer;
@@ -471,31 +485,26 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     bool done = false;
     while (res > p->tolerance) {
         timer.before();
-        RC_TRY(half_sweep(j, sweep & 1, phase32, omega, hits));
+        const bool in32 = st.fp32;
+        RC_TRY(half_sweep(j, st, sweep & 1, omega, hits));
         timer.after();
         sweep++;
-        if (phase32) r.sweeps_fp32++;
+        if (in32) r.sweeps_fp32++;
         if (trace && sweep <= trace_len) {
-            RC_TRY(obs_factored(j, phase32, ob));
+            RC_TRY(obs_factored(j, st.fp32, ob));
             trace[sweep - 1] = ob[which];
         }
-        if (phase32 && sweep != due && sweep % o.mixed_probe_every == 0) {
+        if (st.fp32 && sweep != due && sweep % o.mixed_probe_every == 0) {
             RC_TRY(obs_factored(j, true, ob));
             r.residue_evals++;
-            if (ob[which] <= o.mixed_switch) {
-                RC_TRY(promote_g(j));
-                phase32 = false;
-            }
+            if (ob[which] <= o.mixed_switch) RC_TRY(promote_g(j, st));
         }
         if (sweep == due) {
             const double last = res;
-            RC_TRY(obs_factored(j, phase32, ob));
+            RC_TRY(obs_factored(j, st.fp32, ob));
             r.residue_evals++;
             res = ob[which];
-            if (phase32 && res <= o.mixed_switch) {
-                RC_TRY(promote_g(j));
-                phase32 = false;
-            }
+            if (st.fp32 && res <= o.mixed_switch) RC_TRY(promote_g(j, st));
             if (std::fabs(res - last) / last < 10E-16) {
                 if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", sweep, res);
                 rc = GFC_ENOTCONVERGED;
@@ -513,15 +522,13 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
             done = true;
             break;
         }
-        if (!((unsigned)sweep % p->sweeps_to_reunitarization)) RC_TRY(reunit_g(j, phase32));
+        if (!((unsigned)sweep % p->sweeps_to_reunitarization))
+            RC_TRY(reunit_g(j, st.fp32, 0, 2));
     }
-    if (phase32) {
-        // left the loop in the FP32 phase (tolerance reached there or max
-        // sweeps): hand the state over so that G and U are FP64
-        RC_TRY(promote_g(j));
-        phase32 = false;
-    }
-    if (!done) RC_TRY(reunit_g(j, false));
+    // left the loop in the FP32 phase (tolerance reached there or max sweeps):
+    // hand the state over so that G and U are FP64
+    if (st.fp32) RC_TRY(promote_g(j, st));
+    if (!done) RC_TRY(reunit_g(j, false, 0, 2));
     RC_TRY(materialize(j));
     HIP_TRY(hipStreamSynchronize(j->ctx->stream));
     r.seconds_sweeps = timer.seconds();

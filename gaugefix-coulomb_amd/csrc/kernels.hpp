@@ -2,10 +2,12 @@
 //
 // Device state of one lattice (V = ns^3 nt sites, Vh = V/2 per parity):
 //   cfg : AoS FP64, the reference's U image (4V matrices, site-major, mu=T,Z,Y,X)
-//   u0  : SoA spatial links of the configuration being fixed, one plane of Vh
-//         complex values per (parity p, direction i = Z,Y,X, element k):
-//         plane index (p*3 + i)*9 + k
-//   g   : SoA gauge transformation, plane index p*9 + k
+//   u0  : SoA spatial links of the configuration being fixed
+//   g   : SoA gauge transformation
+// u0 and g hold SU(3) matrices as their first two rows (6 complex numbers);
+// row 2 = conj(r0 x r1) is rebuilt in registers on every load.  One plane of Vh
+// complex values per (parity p, [direction i = Z,Y,X,] element k < 6):
+//   u0 plane (p*3 + i)*6 + k,   g plane p*6 + k.
 // Within a parity the index is h = lex >> 1 with lex = ((t ns + z) ns + y) ns + x
 // (ns even), so the y/z neighbours of a site sit at a constant offset +-ns/2,
 // +-ns^2/2 in the other parity's planes and the x neighbours at h or h+-1:
@@ -23,6 +25,7 @@
 namespace gfc {
 
 constexpr int kBlock = 256;
+constexpr int kPlanes = 6;  // stored complex elements per SU(3) matrix
 
 // Workgroups are dispatched round-robin over the 8 XCDs (one L2 each); remap
 // so that each XCD walks one contiguous range of sites and the neighbour
@@ -34,18 +37,48 @@ __device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
     return x * q + (x < r ? x : r) + i;
 }
 
-template <typename T>
-__device__ __forceinline__ void load_m(mat3<T> &m, const cplx<T> *__restrict__ plane0,
-                                       int64_t stride, int idx) {
-#pragma unroll
-    for (int k = 0; k < 9; k++) m.e[k] = plane0[k * stride + idx];
+// Loads that stream (read once per sweep) carry the non-temporal hint so that
+// the L2 keeps the neighbour G matrices, which six sites read.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ cplx<double> ld_stream(const cplx<double> *p) {
+    dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p));
+    return c_make<double>(v.x, v.y);
+}
+__device__ __forceinline__ cplx<float> ld_stream(const cplx<float> *p) {
+    fvec2 v = __builtin_nontemporal_load(reinterpret_cast<const fvec2 *>(p));
+    return c_make<float>(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(cplx<double> *p, const cplx<double> &c) {
+    dvec2 v = {c.re, c.im};
+    __builtin_nontemporal_store(v, reinterpret_cast<dvec2 *>(p));
+}
+__device__ __forceinline__ void st_stream(cplx<float> *p, const cplx<float> &c) {
+    fvec2 v = {c.re, c.im};
+    __builtin_nontemporal_store(v, reinterpret_cast<fvec2 *>(p));
 }
 
-template <typename T>
-__device__ __forceinline__ void store_m(const mat3<T> &m, cplx<T> *__restrict__ plane0,
-                                        int64_t stride, int idx) {
+// SU(3) matrix from its two stored rows.
+template <bool STREAM, typename T>
+__device__ __forceinline__ void load_su3(mat3<T> &m, const cplx<T> *__restrict__ plane0,
+                                         int64_t stride, int idx) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) plane0[k * stride + idx] = m.e[k];
+    for (int k = 0; k < kPlanes; k++)
+        m.e[k] = STREAM ? ld_stream(plane0 + k * stride + idx) : plane0[k * stride + idx];
+    m_fill_row2(m);
+}
+
+template <bool STREAM, typename T>
+__device__ __forceinline__ void store_su3(const mat3<T> &m, cplx<T> *__restrict__ plane0,
+                                          int64_t stride, int idx) {
+#pragma unroll
+    for (int k = 0; k < kPlanes; k++) {
+        if (STREAM)
+            st_stream(plane0 + k * stride + idx, m.e[k]);
+        else
+            plane0[k * stride + idx] = m.e[k];
+    }
 }
 
 // Site geometry of parity index h on parity p.
@@ -99,32 +132,33 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0
     const Site s = site_of(h, p, ns);
     const Nbr nb = neighbours(s, h, ns);
     const int64_t st = Vh;
-    const cplx<T> *gq = g + (int64_t)(1 - p) * 9 * st;  // other parity's G
+    const cplx<T> *gq = g + (int64_t)(1 - p) * kPlanes * st;  // other parity's G
     mat3<T> W;
     m_zero(W);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         mat3<T> uf, gf, ub, gb;
-        load_m(uf, u0 + (int64_t)(p * 3 + i) * 9 * st, st, h);
-        load_m(gf, gq, st, nb.fwd[i]);
-        load_m(ub, u0 + (int64_t)((1 - p) * 3 + i) * 9 * st, st, nb.bwd[i]);
-        load_m(gb, gq, st, nb.bwd[i]);
+        load_su3<true>(uf, u0 + (int64_t)(p * 3 + i) * kPlanes * st, st, h);
+        load_su3<false>(gf, gq, st, nb.fwd[i]);
+        load_su3<true>(ub, u0 + (int64_t)((1 - p) * 3 + i) * kPlanes * st, st, nb.bwd[i]);
+        load_su3<false>(gb, gq, st, nb.bwd[i]);
         m_mac_nd(W, uf, gf);  // u0_i(n) G(n+i)^dagger
         m_mac_dd(W, ub, gb);  // u0_i(n-i)^dagger G(n-i)^dagger
     }
-    cplx<T> *gp = g + (int64_t)p * 9 * st;
+    cplx<T> *gp = g + (int64_t)p * kPlanes * st;
     mat3<T> gn, w, upd, out;
-    load_m(gn, gp, st, h);
+    load_su3<true>(gn, gp, st, h);
     m_mul(gn, W, w);
     la_or_update(w, hits, omega, upd);
     m_mul(upd, gn, out);
-    store_m(out, gp, st, h);
+    store_su3<true>(out, gp, st, h);
 }
 
 // ---------------------------------------------------------------------------
-// G <- identity on both parities (setFieldToIdentity, src/fields.c:64-100).
-// The G kernels below take one thread per site over both parities:
-// idx in [0, 2 Vh), parity p = idx >= Vh, plane (p*9 + k) of length Vh.
+// Per-site G kernels: one thread per site of the parities [p0, p0 + np),
+// idx in [0, np Vh), parity p = p0 + idx / Vh.
+
+// G <- identity (setFieldToIdentity, src/fields.c:64-100).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_g_identity(cplx<T> *__restrict__ g, int Vh) {
     const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -132,42 +166,43 @@ __global__ __launch_bounds__(kBlock) void k_g_identity(cplx<T> *__restrict__ g, 
     const int p = idx >= Vh, h = idx - p * Vh;
     mat3<T> m;
     m_identity(m);
-    store_m(m, g + (int64_t)p * 9 * Vh, Vh, h);
+    store_su3<false>(m, g + (int64_t)p * kPlanes * Vh, Vh, h);
 }
 
-// Projects every G onto SU(3) (reunitarizeField on G, src/fields.c:144-187).
-// Only rows 0 and 1 are read: the projection recomputes row 2.
+// Projects G onto SU(3) on parities [p0, p0 + np) (reunitarizeField on G,
+// src/fields.c:144-187).  The projection reads rows 0 and 1 only.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_g_reunit(cplx<T> *__restrict__ g, int Vh) {
+__global__ __launch_bounds__(kBlock) void k_g_reunit(cplx<T> *__restrict__ g, int Vh, int p0,
+                                                     int np) {
     const int idx = blockIdx.x * kBlock + threadIdx.x;
-    if (idx >= 2 * Vh) return;
-    const int p = idx >= Vh, h = idx - p * Vh;
-    cplx<T> *base = g + (int64_t)p * 9 * Vh;
+    if (idx >= np * Vh) return;
+    const int p = p0 + idx / Vh, h = idx % Vh;
+    cplx<T> *base = g + (int64_t)p * kPlanes * Vh;
     mat3<T> m;
 #pragma unroll
-    for (int k = 0; k < 6; k++) m.e[k] = base[(int64_t)k * Vh + h];
+    for (int k = 0; k < kPlanes; k++) m.e[k] = base[(int64_t)k * Vh + h];
     m_project_su3(m);
-    store_m(m, base, Vh, h);
+    store_su3<false>(m, base, Vh, h);
 }
 
-// FP32 -> FP64 hand-off of G: promote then project in FP64.
+// FP32 -> FP64 hand-off of G: promote rows 0, 1 then project in FP64.
 __global__ __launch_bounds__(kBlock) void k_g_promote(const cplx<float> *__restrict__ gf,
                                                       cplx<double> *__restrict__ g, int Vh) {
     const int idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= 2 * Vh) return;
     const int p = idx >= Vh, h = idx - p * Vh;
-    const cplx<float> *src = gf + (int64_t)p * 9 * Vh;
+    const cplx<float> *src = gf + (int64_t)p * kPlanes * Vh;
     mat3<double> m;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < kPlanes; k++) {
         cplx<float> v = src[(int64_t)k * Vh + h];
         m.e[k] = c_make<double>(v.re, v.im);
     }
     m_project_su3(m);
-    store_m(m, g + (int64_t)p * 9 * Vh, Vh, h);
+    store_su3<false>(m, g + (int64_t)p * kPlanes * Vh, Vh, h);
 }
 
-// cfg (AoS FP64) spatial links -> u0 planes of type T.
+// cfg (AoS FP64) spatial links -> u0 planes of type T (rows 0 and 1).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_build_u0(const cplx<double> *__restrict__ cfg,
                                                      cplx<T> *__restrict__ u0, int ns,
@@ -181,9 +216,9 @@ __global__ __launch_bounds__(kBlock) void k_build_u0(const cplx<double> *__restr
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const cplx<double> *src = cfg + (lex * 4 + 1 + i) * 9;
-        cplx<T> *dst = u0 + (int64_t)(p * 3 + i) * 9 * st;
+        cplx<T> *dst = u0 + (int64_t)(p * 3 + i) * kPlanes * st;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
+        for (int k = 0; k < kPlanes; k++) {
             cplx<double> v = src[k];
             dst[k * st + h] = c_make<T>((T)v.re, (T)v.im);
         }
@@ -206,8 +241,8 @@ __global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict
     const int p = (x + y + z + t) & 1;
     const int64_t st = Vh;
     mat3<double> gn;
-    load_m(gn, g + (int64_t)p * 9 * st, st, (int)(lex >> 1));
-    const cplx<double> *gq = g + (int64_t)(1 - p) * 9 * st;
+    load_su3<false>(gn, g + (int64_t)p * kPlanes * st, st, (int)(lex >> 1));
+    const cplx<double> *gq = g + (int64_t)(1 - p) * kPlanes * st;
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
         int c[4] = {t, z, y, x};
@@ -215,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict
         if (c[mu] == (mu == 0 ? nt : ns)) c[mu] = 0;
         const int64_t ln = (((int64_t)c[0] * ns + c[1]) * ns + c[2]) * ns + c[3];
         mat3<double> gm, u, tmp, out;
-        load_m(gm, gq, st, (int)(ln >> 1));
+        load_su3<false>(gm, gq, st, (int)(ln >> 1));
         cplx<double> *ptr = cfg + (lex * 4 + mu) * 9;
 #pragma unroll
         for (int k = 0; k < 9; k++) u.e[k] = ptr[k];
@@ -247,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_f32_to_f64(const float *__restrict__
     if (i < n) out[i] = (double)in[i];
 }
 
-// SoA G (FP64) -> AoS image (site-major).
+// SoA G (FP64) -> AoS image (site-major, full 3x3).
 __global__ __launch_bounds__(kBlock) void k_g_to_aos(const cplx<double> *__restrict__ g,
                                                      cplx<double> *__restrict__ out, int ns,
                                                      int Vh) {
@@ -261,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_g_to_aos(const cplx<double> *__restr
     const int t = (int)(r / ns);
     const int p = (x + y + z + t) & 1;
     mat3<double> m;
-    load_m(m, g + (int64_t)p * 9 * Vh, Vh, (int)(lex >> 1));
+    load_su3<false>(m, g + (int64_t)p * kPlanes * Vh, Vh, (int)(lex >> 1));
 #pragma unroll
     for (int k = 0; k < 9; k++) out[lex * 9 + k] = m.e[k];
 }
@@ -319,13 +354,14 @@ __device__ __forceinline__ Obs site_obs(mat3<double> &D, double f) {
 }
 
 template <typename T>
-__device__ __forceinline__ void load_d(mat3<double> &m, const cplx<T> *__restrict__ plane0,
-                                       int64_t stride, int idx) {
+__device__ __forceinline__ void load_su3_d(mat3<double> &m, const cplx<T> *__restrict__ plane0,
+                                           int64_t stride, int idx) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < kPlanes; k++) {
         const cplx<T> v = plane0[k * stride + idx];
         m.e[k] = c_make<double>((double)v.re, (double)v.im);
     }
+    m_fill_row2(m);
 }
 
 // Deterministic block reduction of three doubles; writes part[3*block + j].
@@ -366,23 +402,23 @@ __global__ __launch_bounds__(kBlock) void k_obs_factored(const cplx<TU> *__restr
         const Site s = site_of(h, p, ns);
         const Nbr nb = neighbours(s, h, ns);
         const int64_t st = Vh;
-        const cplx<TG> *gq = g + (int64_t)(1 - p) * 9 * st;
+        const cplx<TG> *gq = g + (int64_t)(1 - p) * kPlanes * st;
         mat3<double> gn;
-        load_d(gn, g + (int64_t)p * 9 * st, st, h);
+        load_su3_d(gn, g + (int64_t)p * kPlanes * st, st, h);
         mat3<double> D;
         m_zero(D);
         double f = 0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             mat3<double> u, gm, tmp, lk;
-            load_d(u, u0 + (int64_t)(p * 3 + i) * 9 * st, st, h);
-            load_d(gm, gq, st, nb.fwd[i]);
+            load_su3_d(u, u0 + (int64_t)(p * 3 + i) * kPlanes * st, st, h);
+            load_su3_d(gm, gq, st, nb.fwd[i]);
             m_mul(gn, u, tmp);
             m_mul_nd(tmp, gm, lk);  // U_i(n)
             accum_link(D, lk, 1.0);
             f += lk.e[0].re + lk.e[4].re + lk.e[8].re;
-            load_d(u, u0 + (int64_t)((1 - p) * 3 + i) * 9 * st, st, nb.bwd[i]);
-            load_d(gm, gq, st, nb.bwd[i]);
+            load_su3_d(u, u0 + (int64_t)((1 - p) * 3 + i) * kPlanes * st, st, nb.bwd[i]);
+            load_su3_d(gm, gq, st, nb.bwd[i]);
             m_mul(gm, u, tmp);
             m_mul_nd(tmp, gn, lk);  // U_i(n-i)
             accum_link(D, lk, -1.0);
