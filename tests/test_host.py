@@ -78,9 +78,11 @@ def test_geometry_rules():
     assert chk(4, 8) == 0 and chk(48, 96) == 0
     assert chk(0, 8) == gfc.EGEOMETRY and chk(4, -2) == gfc.EGEOMETRY
     assert chk(5, 8) == gfc.EGEOMETRY and chk(4, 7) == gfc.EGEOMETRY  # odd extents
-    assert gfc.footprint(gfc.Geometry(48, 96)) > 10 * 2 ** 30
-    assert gfc.footprint(gfc.Geometry(48, 96), gfc.PREC_MIXED) > gfc.footprint(
-        gfc.Geometry(48, 96))
+    V = 48 ** 3 * 96
+    # cfg 576 B + u0 3 x 96 B + G 96 B per site; mixed adds the FP32 copies
+    assert gfc.footprint(gfc.Geometry(48, 96)) == 960 * V
+    assert gfc.footprint(gfc.Geometry(48, 96), gfc.PREC_MIXED) == (960 + 192) * V
+    assert gfc.footprint(gfc.Geometry(5, 8)) == -1
 
 
 def test_loader_matches_reference(tmp_path):
