@@ -139,6 +139,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--serve", type=int, default=256, metavar="N",
+                    help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--profile-kernels", type=int, default=0, metavar="N",
                     help="only run N FP64 and N FP32 half-sweeps plus a 1 GiB device copy "
                          "(calibration for rocprofv3 PMC counters), then exit")
@@ -201,35 +204,67 @@ def main():
     achieved = BYTES_PER_UPDATE_FP64 * vh / (sweep_ms * 1e-3) / 1e9
     log(f"fp64 sweep kernel: {sweep_ms:.4f} ms/launch, {achieved:.1f} GB/s algorithmic")
 
-    out_mixed = {}
-    if not a.no_mixed:
-        job.restore_config()
-        job.fix(params, mixed)
-        ts, m_sweeps, m32 = [], [], []
-        for i in range(a.steps):
+    def time_fix(prm, opt, reps):
+        ts, last = [], None
+        for _ in range(reps):
             job.restore_config()
             tm = time.perf_counter()
-            rc, rep = job.fix(params, mixed)
+            rc, last = job.fix(prm, opt)
             ts.append(time.perf_counter() - tm)
             assert rc > 0, rc
-            m_sweeps.append(rc)
-            m32.append(rep.sweeps_fp32)
+        return float(np.mean(ts)), last
+
+    out_mixed = {}
+    if not a.no_mixed:
+        time_fix(params, mixed, 1)  # warm-up
+        mixed_s, rep_m = time_fix(params, mixed, a.steps)
         e2m, thm, Fm = job.observables()
         plaqm = job.plaquette()
+        # FP32 sweep-kernel duration (HIP events), 50 FP32 half-sweeps
         job.restore_config()
-        rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_MIXED, time_sweeps=1))
-        ms32 = None
-        mixed_s = float(np.mean(ts))
+        rc, rep32 = job.fix(gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
+                                       max_sweeps_to_fix=50, sweeps_to_reunitarization=10),
+                            gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0,
+                                        mixed_probe_every=10 ** 6, time_sweeps=1))
+        sweep32_ms = rep32.seconds_sweeps / rep32.sweeps * 1e3
+        # the same comparison under the dense check policy (a check every 10 sweeps)
+        dense = gfc.Params(tolerance=1e-16, estimate_sweeps_to_gf_progress=0,
+                           sweeps_to_reunitarization=10, omega_OR=1.95, hits=2)
+        d64_s, d64 = time_fix(dense, fp64, 1)
+        dmx_s, dmx = time_fix(dense, mixed, 1)
         out_mixed = {
             "mixed_seconds": round(mixed_s, 4),
-            "mixed_sweeps": m_sweeps[0], "mixed_sweeps_fp32": m32[0],
+            "mixed_sweeps": rep_m.sweeps, "mixed_sweeps_fp32": rep_m.sweeps_fp32,
             "mixed_speedup_vs_fp64": round(step_s / mixed_s, 3),
-            "mixed_final_e2": e2m, "mixed_F": Fm, "mixed_F_rel_diff_vs_fp64": abs(Fm - F64) / abs(F64),
+            "mixed_final_e2": e2m, "mixed_F": Fm,
+            "mixed_F_rel_diff_vs_fp64": abs(Fm - F64) / abs(F64),
             "mixed_plaquette_drift": max(abs(plaqm[0] - plaq0[0]), abs(plaqm[1] - plaq0[1])),
+            "fp32_sweep_kernel_ms": round(sweep32_ms, 5),
+            "fp32_sweep_GBs_algorithmic": round(BYTES_PER_UPDATE_FP32 * vh / sweep32_ms / 1e6, 1),
+            "dense_schedule": {"fp64_seconds": round(d64_s, 4), "fp64_sweeps": d64.sweeps,
+                               "mixed_seconds": round(dmx_s, 4), "mixed_sweeps": dmx.sweeps,
+                               "mixed_sweeps_fp32": dmx.sweeps_fp32,
+                               "mixed_speedup_vs_fp64": round(d64_s / dmx_s, 3)},
         }
-        log(f"mixed: {mixed_s:.3f}s sweeps={m_sweeps} fp32={m32} e2={e2m:.3e} "
-            f"speedup={step_s / mixed_s:.3f}")
+        log(f"mixed: {mixed_s:.3f}s sweeps={rep_m.sweeps} fp32={rep_m.sweeps_fp32} e2={e2m:.3e} "
+            f"speedup={step_s / mixed_s:.3f}; fp32 sweep {sweep32_ms:.4f} ms; dense schedule: "
+            f"fp64 {d64_s:.3f}s ({d64.sweeps}) mixed {dmx_s:.3f}s ({dmx.sweeps}, "
+            f"fp32 {dmx.sweeps_fp32}) speedup {d64_s / dmx_s:.3f}")
     job.close()
+
+    serve_out = None
+    if a.serve > 0:
+        from gfc import serve
+        reqs = serve.stream_requests(n=a.serve, rate=a.serve_rate)
+        log(f"serving {a.serve} requests, Poisson rate {a.serve_rate}/s, mixed precision")
+        with gfc.Server(ctx, max_jobs=64) as srv:
+            serve_out, _ = serve.run_stream(srv, reqs, gfc.Options(precision=gfc.PREC_MIXED),
+                                            log=log)
+        serve_out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in serve_out.items()}
+        serve_out["workload"] = ("config 4: sizes {16^3x32: 40%, 24^3x48: 30%, 32^3x64: 20%, "
+                                 "48^3x96: 10%}, eps U[0.2,0.4], tol {1e-14, 1e-16}, "
+                                 f"Poisson {a.serve_rate}/s, inputs synthesised in HBM")
+        log(f"serve: {serve_out}")
     ctx.close()
 
     cpu = None
@@ -267,6 +302,7 @@ def main():
                      "traffic": pmc_traffic(a.ns, a.nt, "fp64")},
         "copy_kernel_GBs": round(copy_gbs, 1),
         "sweep_frac_of_copy": round(achieved / copy_gbs, 4),
+        "serve": serve_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -15,61 +15,20 @@
 #include <string>
 #include <vector>
 
-#include "gfc.h"
-#include "kernels.hpp"
+#include "internal.hpp"
 
 using gfc::cplx;
 using gfc::kBlock;
+using gfcx::fail;
+using gfcx::nblocks;
 
-namespace {
+thread_local std::string gfcx::last_error;
 
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string &msg) {
-    g_last_error = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                 \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            return fail(GFC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
-
-inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
-
-}  // namespace
-
-struct gfc_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    double *part = nullptr;  // per-block partial sums
-    size_t part_cap = 0;     // in blocks
-    double *res_dev = nullptr;
-    double *res_host = nullptr;  // pinned, 4 doubles
-    int *flag_dev = nullptr;
-};
-
-struct gfc_job {
-    gfc_ctx *ctx = nullptr;
-    gfc_geom geom{0, 0};
-    int64_t V = 0;
-    int Vh = 0;
-    int precision = GFC_PREC_FP64;
-    cplx<double> *cfg = nullptr;  // AoS FP64 U, 4V matrices
-    cplx<double> *u0 = nullptr;   // SoA spatial links, FP64
-    cplx<double> *g = nullptr;    // SoA G, FP64
-    cplx<float> *u0f = nullptr;   // SoA spatial links, FP32 (MIXED)
-    cplx<float> *gf = nullptr;    // SoA G, FP32 (MIXED)
-    cplx<double> *saved = nullptr;  // gfc_job_save_config copy of cfg
-};
+using namespace gfcx;
 
 extern "C" {
 
-const char *gfc_last_error(void) { return g_last_error.c_str(); }
+const char *gfc_last_error(void) { return gfcx::last_error.c_str(); }
 
 const char *gfc_version(void) { return "gfc-mi355x 0.1 (gfx950)"; }
 
@@ -79,6 +38,7 @@ void gfc_options_default(gfc_options *o) {
     o->mixed_probe_every = 50;
     o->verbose = 0;
     o->time_sweeps = 0;
+    o->retire_slices = 0;
 }
 
 int gfc_ctx_create(int device, gfc_ctx **out) {
@@ -176,7 +136,7 @@ void gfc_job_destroy(gfc_job *j) {
 
 }  // extern "C"
 
-namespace {
+namespace gfcx {
 
 int ensure_partials(gfc_ctx *c, size_t blocks) {
     if (blocks <= c->part_cap) return GFC_OK;
@@ -266,6 +226,31 @@ extern "C" int gfc_job_upload_config(gfc_job *j, const void *U, int in_precision
     return GFC_OK;
 }
 
+int gfcx::generate_smooth_device(gfc_job *j, uint64_t seed, double eps) {
+    hipStream_t s = j->ctx->stream;
+    // base state of the draws, as in host_gen.c: mix64(seed + golden gamma)
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    const uint64_t base = z ^ (z >> 31);
+    // g(n) staging in the u0 planes (288V bytes >= 144V bytes needed)
+    cplx<double> *tmp = j->u0;
+    hipLaunchKernelGGL(gfc::k_gen_haar, dim3(nblocks(j->V)), dim3(kBlock), 0, s, base, j->V, tmp);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(gfc::k_gen_links, dim3(nblocks(j->V)), dim3(kBlock), 0, s, base, eps,
+                       j->geom.ns, j->geom.nt, tmp, j->cfg);
+    LAUNCH_CHECK();
+    return set_identity(j);
+}
+
+extern "C" int gfc_job_generate_smooth(gfc_job *j, uint64_t seed, double eps) {
+    if (j == nullptr) return fail(GFC_EINVAL, "gfc_job_generate_smooth: NULL");
+    HIP_TRY(hipSetDevice(j->ctx->device));
+    RC_TRY(generate_smooth_device(j, seed, eps));
+    HIP_TRY(hipStreamSynchronize(j->ctx->stream));
+    return GFC_OK;
+}
+
 extern "C" int gfc_job_reunitarize(gfc_job *j) {
     if (j == nullptr) return fail(GFC_EINVAL, "gfc_job_reunitarize: NULL");
     HIP_TRY(hipSetDevice(j->ctx->device));
@@ -282,7 +267,7 @@ extern "C" int gfc_job_reunitarize(gfc_job *j) {
     return GFC_OK;
 }
 
-namespace {
+namespace gfcx {
 
 // Reduces the per-block partials of the last observable kernel and returns the
 // three sums on the host.
@@ -401,11 +386,11 @@ int promote_g(gfc_job *j, SweepState &st) {
     return GFC_OK;
 }
 
-#define RC_TRY(expr)          \
-    do {                      \
-        int rc_ = (expr);     \
-        if (rc_) return rc_;  \
-    } while (0)
+int promote_g_only(gfc_job *j) {
+    SweepState st;
+    st.fp32 = true;
+    return promote_g(j, st);
+}
 
 struct SweepTimer {
     bool on = false;
@@ -444,6 +429,115 @@ struct SweepTimer {
     }
 };
 
+// Per-timeslice convergence tracking (gfc_options.retire_slices).  w, e2 and
+// theta couple only sites of one timeslice, so a slice whose own proxy is at or
+// below the tolerance can stop being swept: its proxy stays frozen while the
+// others keep converging, and the global proxy (the mean over slices) is
+// still what the check schedule tests.  Active slices are swept with the
+// batched sweep kernel over (h0, parity) items of those slices only.
+struct SliceTracker {
+    bool on = false;
+    int nt = 0, S = 0;  // S = ns^3 / 2 parity sites per timeslice
+    std::vector<char> active;
+    int n_active = 0;
+    double slice_sweeps = 0;
+    gfc::BatchJob *d_job = nullptr;
+    int4 *d_items[2] = {nullptr, nullptr};
+    int n_items[2] = {0, 0};
+    bool dirty = true;
+    bool desc_fp32 = false;
+    std::vector<double> part;
+    ~SliceTracker() {
+        hipFree(d_job);
+        hipFree(d_items[0]);
+        hipFree(d_items[1]);
+    }
+    int init(gfc_job *j) {
+        on = true;
+        nt = j->geom.nt;
+        S = j->geom.ns * j->geom.ns * j->geom.ns / 2;
+        active.assign(nt, 1);
+        n_active = nt;
+        HIP_TRY(hipMalloc(&d_job, sizeof(gfc::BatchJob)));
+        for (int p = 0; p < 2; p++)
+            HIP_TRY(hipMalloc(&d_items[p], sizeof(int4) * (size_t)(j->Vh / kBlock + 1)));
+        return GFC_OK;
+    }
+    // Sweeps the active slices of parity p (nothing when all have retired).
+    int sweep(gfc_job *j, bool fp32, int p, double omega, int hits) {
+        hipStream_t s = j->ctx->stream;
+        slice_sweeps += (double)n_active / nt;
+        if (n_active == 0) return GFC_OK;
+        if (dirty || desc_fp32 != fp32) {
+            gfc::BatchJob b{};
+            b.u0 = fp32 ? (const void *)j->u0f : (const void *)j->u0;
+            b.g = fp32 ? (void *)j->gf : (void *)j->g;
+            b.omega = omega;
+            b.ns = j->geom.ns;
+            b.Vh = j->Vh;
+            b.hits = hits;
+            std::vector<int4> it[2];
+            for (int t = 0; t < nt; t++)
+                if (active[t])
+                    for (int q = 0; q < 2; q++)
+                        for (int h0 = t * S; h0 < (t + 1) * S; h0 += kBlock)
+                            it[q].push_back(make_int4(0, h0, q, 0));
+            HIP_TRY(hipMemcpyAsync(d_job, &b, sizeof b, hipMemcpyHostToDevice, s));
+            for (int q = 0; q < 2; q++) {
+                n_items[q] = (int)it[q].size();
+                HIP_TRY(hipMemcpyAsync(d_items[q], it[q].data(), sizeof(int4) * it[q].size(),
+                                       hipMemcpyHostToDevice, s));
+            }
+            HIP_TRY(hipStreamSynchronize(s));
+            dirty = false;
+            desc_fp32 = fp32;
+        }
+        const int n = n_items[p];
+        if (fp32)
+            hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3(n), dim3(kBlock), 0, s, d_job,
+                               d_items[p], n, 1);
+        else
+            hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3(n), dim3(kBlock), 0, s, d_job,
+                               d_items[p], n, 1);
+        LAUNCH_CHECK();
+        return GFC_OK;
+    }
+    // Proxy of every timeslice of the current state; retires those at or below
+    // tol.  Returns the global proxy (mean over slices) in *global.
+    int probe(gfc_job *j, bool fp32, int which, double tol, double *global) {
+        gfc_ctx *c = j->ctx;
+        const unsigned b = nblocks(2 * (int64_t)j->Vh);
+        if (fp32)
+            hipLaunchKernelGGL((gfc::k_obs_factored<double, float>), dim3(b), dim3(kBlock), 0,
+                               c->stream, j->u0, j->gf, j->geom.ns, j->Vh, c->part);
+        else
+            hipLaunchKernelGGL((gfc::k_obs_factored<double, double>), dim3(b), dim3(kBlock), 0,
+                               c->stream, j->u0, j->g, j->geom.ns, j->Vh, c->part);
+        LAUNCH_CHECK();
+        part.resize((size_t)b * 3);
+        HIP_TRY(hipMemcpyAsync(part.data(), c->part, sizeof(double) * part.size(),
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const int bps = S / kBlock, half = j->Vh / kBlock;
+        const double vt = 2.0 * S;  // sites per slice
+        double sum = 0;
+        for (int t = 0; t < nt; t++) {
+            double acc = 0;
+            for (int q = 0; q < 2; q++)
+                for (int k = 0; k < bps; k++) acc += part[3 * (q * half + t * bps + k) + which];
+            const double pt = acc / (which == 1 ? 3.0 * vt : vt);
+            sum += pt;
+            if (active[t] && pt <= tol) {
+                active[t] = 0;
+                n_active--;
+                dirty = true;
+            }
+        }
+        *global = sum / nt;
+        return GFC_OK;
+    }
+};
+
 int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report *rep,
             double *trace, int trace_len) {
     if (j == nullptr || p == nullptr) return fail(GFC_EINVAL, "gfc_job_fix: NULL argument");
@@ -473,6 +567,12 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     SweepTimer timer;
     timer.on = o.time_sweeps != 0;
     timer.s = j->ctx->stream;
+    SliceTracker sl;
+    if (o.retire_slices) {
+        if (j->geom.ns % 8)
+            return fail(GFC_EINVAL, "gfc_job_fix: retire_slices needs ns a multiple of 8");
+        RC_TRY(sl.init(j));
+    }
 
     RC_TRY(obs_factored(j, false, ob));
     r.residue_evals++;
@@ -486,7 +586,10 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     while (res > p->tolerance) {
         timer.before();
         const bool in32 = st.fp32;
-        RC_TRY(half_sweep(j, st, sweep & 1, omega, hits));
+        if (sl.on)
+            RC_TRY(sl.sweep(j, st.fp32, sweep & 1, omega, hits));
+        else
+            RC_TRY(half_sweep(j, st, sweep & 1, omega, hits));
         timer.after();
         sweep++;
         if (in32) r.sweeps_fp32++;
@@ -494,10 +597,16 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
             RC_TRY(obs_factored(j, st.fp32, ob));
             trace[sweep - 1] = ob[which];
         }
-        if (st.fp32 && sweep != due && sweep % o.mixed_probe_every == 0) {
-            RC_TRY(obs_factored(j, true, ob));
+        if ((st.fp32 || sl.on) && sweep != due && sweep % o.mixed_probe_every == 0) {
+            double g = 0;
+            if (sl.on) {
+                RC_TRY(sl.probe(j, st.fp32, which, p->tolerance, &g));
+            } else {
+                RC_TRY(obs_factored(j, true, ob));
+                g = ob[which];
+            }
             r.residue_evals++;
-            if (ob[which] <= o.mixed_switch) RC_TRY(promote_g(j, st));
+            if (st.fp32 && g <= o.mixed_switch) RC_TRY(promote_g(j, st));
         }
         if (sweep == due) {
             const double last = res;
@@ -533,11 +642,12 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     HIP_TRY(hipStreamSynchronize(j->ctx->stream));
     r.seconds_sweeps = timer.seconds();
     r.sweeps = sweep;
+    r.slice_sweeps = sl.on ? sl.slice_sweeps : (double)sweep;
     r.residue_final = res;
     r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rep) *rep = r;
     if (rc == GFC_ENOTCONVERGED) {
-        g_last_error = "gfc_job_fix: not converged (max sweeps or stalled residue)";
+        gfcx::last_error = "gfc_job_fix: not converged (max sweeps or stalled residue)";
         return rc;
     }
     return sweep;

@@ -1,0 +1,79 @@
+// internal.hpp -- host-side internals shared by the engine translation units
+// (engine.hip: single-lattice jobs; server.hip: the serving scheduler).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "gfc.h"
+#include "kernels.hpp"
+
+namespace gfcx {
+
+using gfc::cplx;
+using gfc::kBlock;
+
+// Message of the most recent failure of the calling thread (gfc_last_error).
+extern thread_local std::string last_error;
+
+inline int fail(int code, const std::string &msg) {
+    last_error = msg;
+    return code;
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace gfcx
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return gfcx::fail(GFC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+#define RC_TRY(expr)         \
+    do {                     \
+        int rc_ = (expr);    \
+        if (rc_) return rc_; \
+    } while (0)
+
+struct gfc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *part = nullptr;  // per-block partial sums
+    size_t part_cap = 0;     // in blocks
+    double *res_dev = nullptr;
+    double *res_host = nullptr;  // pinned, 4 doubles
+    int *flag_dev = nullptr;
+};
+
+struct gfc_job {
+    gfc_ctx *ctx = nullptr;
+    gfc_geom geom{0, 0};
+    int64_t V = 0;
+    int Vh = 0;
+    int precision = GFC_PREC_FP64;
+    gfc::cplx<double> *cfg = nullptr;    // AoS FP64 U, 4V matrices
+    gfc::cplx<double> *u0 = nullptr;     // SoA spatial links, FP64 (two rows)
+    gfc::cplx<double> *g = nullptr;      // SoA G, FP64 (two rows)
+    gfc::cplx<float> *u0f = nullptr;     // SoA spatial links, FP32 (MIXED)
+    gfc::cplx<float> *gf = nullptr;      // SoA G, FP32 (MIXED)
+    gfc::cplx<double> *saved = nullptr;  // gfc_job_save_config copy of cfg
+};
+
+namespace gfcx {
+
+// Helpers of engine.hip used by the scheduler.  All launch on j->ctx->stream.
+int set_identity(gfc_job *j);
+int build_u0(gfc_job *j);
+int materialize(gfc_job *j);
+int reunit_g(gfc_job *j, bool fp32, int p0, int np);
+int promote_g_only(gfc_job *j);
+int generate_smooth_device(gfc_job *j, uint64_t seed, double eps);
+int ensure_partials(gfc_ctx *c, size_t blocks);
+
+}  // namespace gfcx

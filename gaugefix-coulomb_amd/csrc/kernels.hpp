@@ -27,6 +27,10 @@ namespace gfc {
 constexpr int kBlock = 256;
 constexpr int kPlanes = 6;  // stored complex elements per SU(3) matrix
 
+// Kernels and device helpers have internal linkage: each translation unit
+// that launches them (engine.hip, server.hip) carries its own code object.
+namespace {
+
 // Workgroups are dispatched round-robin over the 8 XCDs (one L2 each); remap
 // so that each XCD walks one contiguous range of sites and the neighbour
 // reads of adjacent z-planes hit the same L2.
@@ -119,16 +123,13 @@ __device__ __forceinline__ Nbr neighbours(const Site &s, int h, int ns) {
 }
 
 // ---------------------------------------------------------------------------
-// One red/black half-sweep on parity p over parity indices [h0, h0 + count).
-// Reads 6 u0 links, 6 neighbour G and G(n); writes G(n).
+// The site update of parity index h on parity p: reads 6 u0 links, 6
+// neighbour G and G(n); writes G(n).  Shared by the single-lattice and the
+// batched (serving) kernels, so a site computes the same bits in both.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0,
-                                                  cplx<T> *__restrict__ g, int ns, int Vh,
-                                                  int p, T omega, int hits, int h0,
-                                                  int count) {
-    const int tid = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
-    if (tid >= count) return;
-    const int h = h0 + tid;
+__device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
+                                           cplx<T> *__restrict__ g, int ns, int Vh, int p,
+                                           T omega, int hits, int h) {
     const Site s = site_of(h, p, ns);
     const Nbr nb = neighbours(s, h, ns);
     const int64_t st = Vh;
@@ -154,6 +155,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0
     store_su3<true>(out, gp, st, h);
 }
 
+// One red/black half-sweep on parity p over parity indices [h0, h0 + count).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0,
+                                                  cplx<T> *__restrict__ g, int ns, int Vh,
+                                                  int p, T omega, int hits, int h0,
+                                                  int count) {
+    const int tid = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+    if (tid >= count) return;
+    sweep_site<T>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
+}
+
 // ---------------------------------------------------------------------------
 // Per-site G kernels: one thread per site of the parities [p0, p0 + np),
 // idx in [0, np Vh), parity p = p0 + idx / Vh.
@@ -172,11 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_g_identity(cplx<T> *__restrict__ g, 
 // Projects G onto SU(3) on parities [p0, p0 + np) (reunitarizeField on G,
 // src/fields.c:144-187).  The projection reads rows 0 and 1 only.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_g_reunit(cplx<T> *__restrict__ g, int Vh, int p0,
-                                                     int np) {
-    const int idx = blockIdx.x * kBlock + threadIdx.x;
-    if (idx >= np * Vh) return;
-    const int p = p0 + idx / Vh, h = idx % Vh;
+__device__ __forceinline__ void reunit_site(cplx<T> *__restrict__ g, int Vh, int p, int h) {
     cplx<T> *base = g + (int64_t)p * kPlanes * Vh;
     mat3<T> m;
 #pragma unroll
@@ -185,12 +193,18 @@ __global__ __launch_bounds__(kBlock) void k_g_reunit(cplx<T> *__restrict__ g, in
     store_su3<false>(m, base, Vh, h);
 }
 
-// FP32 -> FP64 hand-off of G: promote rows 0, 1 then project in FP64.
-__global__ __launch_bounds__(kBlock) void k_g_promote(const cplx<float> *__restrict__ gf,
-                                                      cplx<double> *__restrict__ g, int Vh) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_g_reunit(cplx<T> *__restrict__ g, int Vh, int p0,
+                                                     int np) {
     const int idx = blockIdx.x * kBlock + threadIdx.x;
-    if (idx >= 2 * Vh) return;
-    const int p = idx >= Vh, h = idx - p * Vh;
+    if (idx >= np * Vh) return;
+    reunit_site<T>(g, Vh, p0 + idx / Vh, idx % Vh);
+}
+
+// FP32 -> FP64 hand-off of G: promote rows 0, 1 then project in FP64.
+__device__ __forceinline__ void promote_site(const cplx<float> *__restrict__ gf,
+                                             cplx<double> *__restrict__ g, int Vh, int p,
+                                             int h) {
     const cplx<float> *src = gf + (int64_t)p * kPlanes * Vh;
     mat3<double> m;
 #pragma unroll
@@ -200,6 +214,14 @@ __global__ __launch_bounds__(kBlock) void k_g_promote(const cplx<float> *__restr
     }
     m_project_su3(m);
     store_su3<false>(m, g + (int64_t)p * kPlanes * Vh, Vh, h);
+}
+
+__global__ __launch_bounds__(kBlock) void k_g_promote(const cplx<float> *__restrict__ gf,
+                                                      cplx<double> *__restrict__ g, int Vh) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= 2 * Vh) return;
+    const int p = idx >= Vh;
+    promote_site(gf, g, Vh, p, idx - p * Vh);
 }
 
 // cfg (AoS FP64) spatial links -> u0 planes of type T (rows 0 and 1).
@@ -388,14 +410,12 @@ __device__ __forceinline__ void block_reduce3(Obs o, double *__restrict__ part, 
     }
 }
 
-// Observables of the factored state (u0 of type TU, G of type TG); one thread
-// per site, both parities.
+// Observables of the factored state at site index idx in [0, 2 Vh) (parity
+// idx >= Vh); u0 of type TU, G of type TG.
 template <typename TU, typename TG>
-__global__ __launch_bounds__(kBlock) void k_obs_factored(const cplx<TU> *__restrict__ u0,
-                                                         const cplx<TG> *__restrict__ g,
-                                                         int ns, int Vh,
-                                                         double *__restrict__ part) {
-    const int idx = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ Obs obs_site(const cplx<TU> *__restrict__ u0,
+                                        const cplx<TG> *__restrict__ g, int ns, int Vh,
+                                        int idx) {
     Obs o = {0, 0, 0};
     if (idx < 2 * Vh) {
         const int p = idx >= Vh, h = idx - p * Vh;
@@ -425,7 +445,17 @@ __global__ __launch_bounds__(kBlock) void k_obs_factored(const cplx<TU> *__restr
         }
         o = site_obs(D, f);
     }
-    block_reduce3(o, part, blockIdx.x);
+    return o;
+}
+
+// One thread per site, both parities; per-block partial sums in part.
+template <typename TU, typename TG>
+__global__ __launch_bounds__(kBlock) void k_obs_factored(const cplx<TU> *__restrict__ u0,
+                                                         const cplx<TG> *__restrict__ g,
+                                                         int ns, int Vh,
+                                                         double *__restrict__ part) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    block_reduce3(obs_site<TU, TG>(u0, g, ns, Vh, idx), part, blockIdx.x);
 }
 
 // Observables of an AoS FP64 configuration; one thread per site.
@@ -524,8 +554,8 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const double2 *__restrict__ i
 }
 
 // Sums part[3*b + j] over b in fixed order with one block; out[j].
-__global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restrict__ part,
-                                                         int nblocks, double *__restrict__ out) {
+__device__ __forceinline__ void reduce_parts(const double *__restrict__ part, int nblocks,
+                                             double *__restrict__ out) {
     Obs o = {0, 0, 0};
     for (int b = threadIdx.x; b < nblocks; b += kBlock) {
         o.e2 += part[3 * b];
@@ -535,4 +565,234 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restric
     block_reduce3(o, out, 0);
 }
 
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restrict__ part,
+                                                         int nblocks, double *__restrict__ out) {
+    reduce_parts(part, nblocks, out);
+}
+
+// ---------------------------------------------------------------------------
+// Batched kernels for the serving scheduler: many lattices per launch.  A
+// work item is one 256-site chunk of one lattice; the per-site code is the
+// same device function as in the single-lattice kernels, and reductions use
+// the same block decomposition and order, so every lattice gets the same
+// bits alone or packed with others.
+
+struct BatchJob {
+    const void *u0;   // cplx<T> planes of the lattice's u0 (T = launch type)
+    void *g;          // cplx<T> planes of its G
+    const void *u0d;  // FP64 u0 (observables always read FP64 links)
+    const void *gf;   // FP32 G (MIXED), for FP32-state observables
+    double omega;
+    int ns, Vh, hits;
+    int part_off;     // first partial-sum slot of this lattice (observables)
+};
+
+// item = (job index, first parity index, parity, unused)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sweep_batch(const BatchJob *__restrict__ jobs,
+                                                        const int4 *__restrict__ items,
+                                                        int nitems, int per_block) {
+    const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int i1 = min((b + 1) * per_block, nitems);
+    for (int i = b * per_block; i < i1; i++) {
+        const int4 it = items[i];
+        const BatchJob J = jobs[it.x];
+        const int h = it.y + (int)threadIdx.x;
+        if (h < J.Vh)
+            sweep_site<T>(static_cast<const cplx<T> *>(J.u0), static_cast<cplx<T> *>(J.g), J.ns,
+                          J.Vh, it.z, (T)J.omega, J.hits, h);
+    }
+}
+
+// item = (job index, block index within the lattice's [0, 2 Vh) range, fp32 G?, _)
+__global__ __launch_bounds__(kBlock) void k_obs_batch(const BatchJob *__restrict__ jobs,
+                                                      const int4 *__restrict__ items,
+                                                      double *__restrict__ part) {
+    const int4 it = items[blockIdx.x];
+    const BatchJob J = jobs[it.x];
+    const int idx = it.y * kBlock + (int)threadIdx.x;
+    const cplx<double> *u0 = static_cast<const cplx<double> *>(J.u0d);
+    Obs o = it.z ? obs_site<double, float>(u0, static_cast<const cplx<float> *>(J.gf), J.ns,
+                                           J.Vh, idx)
+                 : obs_site<double, double>(u0, static_cast<const cplx<double> *>(J.g), J.ns,
+                                            J.Vh, idx);
+    block_reduce3(o, part + 3 * (int64_t)J.part_off, it.y);
+}
+
+// One block per evaluated lattice: jobs[sel[k]] partials -> out[3k + j].
+__global__ __launch_bounds__(kBlock) void k_reduce_batch(const BatchJob *__restrict__ jobs,
+                                                         const int *__restrict__ sel,
+                                                         const double *__restrict__ part,
+                                                         double *__restrict__ out) {
+    const BatchJob J = jobs[sel[blockIdx.x]];
+    const int nb = (2 * J.Vh + kBlock - 1) / kBlock;
+    reduce_parts(part + 3 * (int64_t)J.part_off, nb, out + 3 * blockIdx.x);
+}
+
+// item = (job index, block index within [0, 2 Vh), _, _): project G
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reunit_batch(const BatchJob *__restrict__ jobs,
+                                                         const int4 *__restrict__ items) {
+    const int4 it = items[blockIdx.x];
+    const BatchJob J = jobs[it.x];
+    const int idx = it.y * kBlock + (int)threadIdx.x;
+    if (idx >= 2 * J.Vh) return;
+    const int p = idx >= J.Vh;
+    reunit_site<T>(static_cast<cplx<T> *>(J.g), J.Vh, p, idx - p * J.Vh);
+}
+
+// ---------------------------------------------------------------------------
+// Device-side synthetic configuration: the same construction as the host
+// generator (host_gen.c: splitmix64 draws, Box-Muller normals, Gram-Schmidt
+// Haar SU(3), exp(i eps H) by scaling and squaring of a 16-term Taylor
+// polynomial, U_mu(n) = g(n) exp(i eps H_mu(n)) g(n+mu)^dagger).  The device
+// libm and FMA contraction make it agree with the host one to rounding, not
+// bit for bit; the serving scheduler uses it to synthesise request inputs.
+
+__device__ __forceinline__ uint64_t gen_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void gen_normals(uint64_t base, uint64_t site, unsigned k,
+                                            double &a, double &b) {
+    const uint64_t gold = 0x9E3779B97F4A7C15ULL;
+    const uint64_t z1 = gen_mix64(base + (site * 64 + k + 1) * gold);
+    const uint64_t z2 = gen_mix64(base + (site * 64 + k + 2) * gold);
+    const double u1 = (double)((z1 >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)((z2 >> 11) + 1) * 0x1.0p-53;
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(6.283185307179586 * u2, &sn, &cs);
+    a = r * cs;
+    b = r * sn;
+}
+
+__device__ void gen_haar(uint64_t base, uint64_t site, mat3<double> &g) {
+    double v[18];
+#pragma unroll
+    for (unsigned k = 0; k < 18; k += 2) gen_normals(base, site, k, v[k], v[k + 1]);
+#pragma unroll
+    for (int k = 0; k < 9; k++) g.e[k] = c_make<double>(v[2 * k], v[2 * k + 1]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int q = 0; q < i; q++) {
+            cplx<double> ip = c_make<double>(0, 0);
+#pragma unroll
+            for (int j = 0; j < 3; j++) c_mac_cb(ip, g.e[3 * i + j], g.e[3 * q + j]);
+            // ip = <r_q, r_i> = sum conj(r_q) r_i
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const cplx<double> rq = g.e[3 * q + j];
+                g.e[3 * i + j].re -= ip.re * rq.re - ip.im * rq.im;
+                g.e[3 * i + j].im -= ip.re * rq.im + ip.im * rq.re;
+            }
+        }
+        double n = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) n += g.e[3 * i + j].re * g.e[3 * i + j].re +
+                                         g.e[3 * i + j].im * g.e[3 * i + j].im;
+        n = 1.0 / sqrt(n);
+#pragma unroll
+        for (int j = 0; j < 3; j++) g.e[3 * i + j] = c_make(g.e[3 * i + j].re * n, g.e[3 * i + j].im * n);
+    }
+    const cplx<double> d = m_det(g);
+    double sn, cs;
+    sincos(-atan2(d.im, d.re) / 3.0, &sn, &cs);
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        g.e[k] = c_make(g.e[k].re * cs - g.e[k].im * sn, g.e[k].re * sn + g.e[k].im * cs);
+}
+
+__device__ void gen_expi(uint64_t base, uint64_t site, int mu, double eps, mat3<double> &E) {
+    double d[10];
+    const unsigned k0 = 18 + 10 * (unsigned)mu;
+#pragma unroll
+    for (unsigned k = 0; k < 10; k += 2) gen_normals(base, site, k0 + k, d[k], d[k + 1]);
+    const double s2 = 0.7071067811865476;
+    const double mean = (d[0] + d[1] + d[2]) / 3.0;
+    mat3<double> H;
+    H.e[0] = c_make(d[0] - mean, 0.0);
+    H.e[4] = c_make(d[1] - mean, 0.0);
+    H.e[8] = c_make(d[2] - mean, 0.0);
+    H.e[1] = c_make(d[3] * s2, d[4] * s2);
+    H.e[2] = c_make(d[5] * s2, d[6] * s2);
+    H.e[5] = c_make(d[7] * s2, d[8] * s2);
+    H.e[3] = c_make(H.e[1].re, -H.e[1].im);
+    H.e[6] = c_make(H.e[2].re, -H.e[2].im);
+    H.e[7] = c_make(H.e[5].re, -H.e[5].im);
+    double nrm = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double rs = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) rs += hypot(H.e[3 * i + j].re, H.e[3 * i + j].im);
+        nrm = rs > nrm ? rs : nrm;
+    }
+    nrm *= fabs(eps);
+    int sq = 0;
+    while (nrm > 0.5 && sq < 64) {
+        nrm *= 0.5;
+        sq++;
+    }
+    const double scale = ldexp(eps, -sq);
+    mat3<double> A, T;
+    // A = i scale H
+#pragma unroll
+    for (int k = 0; k < 9; k++) A.e[k] = c_make(-scale * H.e[k].im, scale * H.e[k].re);
+    m_identity(E);
+    for (int n = 16; n >= 1; n--) {
+        m_mul(A, E, T);
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            E.e[k] = c_make(T.e[k].re / n + ((k % 4 == 0) ? 1.0 : 0.0), T.e[k].im / n);
+    }
+    for (int s = 0; s < sq; s++) {
+        m_mul(E, E, T);
+        E = T;
+    }
+    m_project_su3(E);
+}
+
+// g(n) for every site into tmp (AoS, 9 complex per site).
+__global__ __launch_bounds__(kBlock) void k_gen_haar(uint64_t base, int64_t V,
+                                                     cplx<double> *__restrict__ tmp) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= V) return;
+    mat3<double> g;
+    gen_haar(base, (uint64_t)s, g);
+#pragma unroll
+    for (int k = 0; k < 9; k++) tmp[s * 9 + k] = g.e[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gen_links(uint64_t base, double eps, int ns, int nt,
+                                                      const cplx<double> *__restrict__ tmp,
+                                                      cplx<double> *__restrict__ cfg) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t V = (int64_t)ns * ns * ns * nt;
+    if (s >= V) return;
+    const int x = (int)(s % ns), y = (int)((s / ns) % ns), z = (int)((s / ns / ns) % ns);
+    const int t = (int)(s / ((int64_t)ns * ns * ns));
+    mat3<double> gs;
+#pragma unroll
+    for (int k = 0; k < 9; k++) gs.e[k] = tmp[s * 9 + k];
+    for (int mu = 0; mu < 4; mu++) {
+        int c[4] = {t, z, y, x};
+        c[mu] += 1;
+        if (c[mu] == (mu == 0 ? nt : ns)) c[mu] = 0;
+        const int64_t sn = (((int64_t)c[0] * ns + c[1]) * ns + c[2]) * ns + c[3];
+        mat3<double> X, T1, gn, out;
+        gen_expi(base, (uint64_t)s, mu, eps, X);
+#pragma unroll
+        for (int k = 0; k < 9; k++) gn.e[k] = tmp[sn * 9 + k];
+        m_mul(gs, X, T1);
+        m_mul_nd(T1, gn, out);
+#pragma unroll
+        for (int k = 0; k < 9; k++) cfg[(s * 4 + mu) * 9 + k] = out.e[k];
+    }
+}
+
+}  // namespace
 }  // namespace gfc

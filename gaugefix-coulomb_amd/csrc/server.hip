@@ -1,0 +1,613 @@
+// server.hip -- continuous-batching gauge-fix server (gfc_server_* in gfc.h).
+//
+// Requests (lattice size, seed, eps, parameters, precision) queue on the host.
+// A request is admitted when a job slot and its HBM footprint are free: its
+// configuration is synthesised in HBM, reunitarized, and its fixing state is
+// the one gfc_job_fix keeps (sweep count, check schedule, FP32/FP64 phase).
+// Every poll advances all resident lattices together:
+//   1. admit what fits;
+//   2. one batched residue evaluation for every lattice at a check, a mixed-
+//      precision probe or its start (one host synchronisation);
+//   3. the decisions of gaugefixOverrelaxation (src/gauge_fixing.c:771-820)
+//      per lattice: stall, convergence, next check, max sweeps,
+//      reunitarization; finished lattices are materialised and retired;
+//   4. a chunk of half-sweeps: each lattice runs up to its next event (next
+//      multiple of sweeps_to_reunitarization, next check, next probe, max),
+//      and every half-sweep step is ONE launch per precision that packs the
+//      256-site chunks of all lattices (k_sweep_batch).  The chunk's launch
+//      sequence is captured in a HIP graph and replayed while the plan repeats.
+// Per-site arithmetic and reduction order are those of the single-lattice
+// path, so a lattice gets the same result alone or in the mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "internal.hpp"
+
+using gfc::BatchJob;
+using gfc::cplx;
+using gfc::kBlock;
+using gfcx::fail;
+using gfcx::nblocks;
+
+namespace {
+
+enum State { QUEUED, INIT, RUN, DONE };
+
+struct Req {
+    int64_t id = 0;
+    uint64_t tag = 0;
+    gfc_geom geom{0, 0};
+    uint64_t seed = 0;
+    double eps = 0;
+    gfc_params p{};
+    gfc_options o{};
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    gfc_job *job = nullptr;
+    int64_t footprint = 0;
+    int prec = GFC_PREC_FP64;
+    State state = QUEUED;
+    // fixing state, as in run_fix (engine.hip)
+    int sweep = 0, due = 0, evals = 0, sweeps_fp32 = 0, rc = 0;
+    double res = 0, res0 = 0, F = 0;
+    bool fp32 = false, finished_early = false;
+    int slot = -1;
+    double obs[3] = {0, 0, 0};
+};
+
+template <typename T>
+T *dev_alloc(size_t n) {
+    T *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    return p;
+}
+
+}  // namespace
+
+struct gfc_server {
+    gfc_ctx *ctx = nullptr;
+    int64_t budget = 0, used = 0;
+    int max_jobs = 0, keep = 0;
+    std::chrono::steady_clock::time_point t0;
+    int64_t next_id = 1;
+    std::deque<std::unique_ptr<Req>> queue;
+    std::vector<std::unique_ptr<Req>> active;
+    std::map<int64_t, std::unique_ptr<Req>> kept;
+    std::multimap<std::tuple<int, int, int>, gfc_job *> cache;  // (ns, nt, prec) -> free job
+    std::vector<gfc_completion> done;
+    // device-side scheduling tables
+    BatchJob *d_jobs = nullptr;
+    BatchJob *h_jobs = nullptr;  // pinned
+    int4 *d_items = nullptr, *h_items = nullptr;  // sweep items of the current chunk plan
+    size_t items_cap = 0;
+    int4 *d_oitems = nullptr, *h_oitems = nullptr;  // residue-evaluation items
+    size_t oitems_cap = 0;
+    int *d_sel = nullptr, *h_sel = nullptr;
+    double *d_part = nullptr;
+    size_t part_cap = 0;
+    double *d_out = nullptr, *h_out = nullptr;
+    // chunk graph
+    hipGraphExec_t graph = nullptr;
+    std::vector<int64_t> plan_sig;
+    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+    gfc_server_stats stats{};
+    double clock() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+int64_t footprint_of(const Req &r) { return gfc_job_footprint(&r.geom, r.prec); }
+
+int grow_table(gfc_server *s, int4 *&d, int4 *&h, size_t &cap, size_t n) {
+    if (n <= cap) return GFC_OK;
+    const size_t c = std::max(n, 2 * cap);
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    hipFree(d);
+    hipHostFree(h);
+    d = nullptr;
+    h = nullptr;
+    cap = 0;
+    HIP_TRY(hipMalloc(&d, c * sizeof(int4)));
+    HIP_TRY(hipHostMalloc(&h, c * sizeof(int4), hipHostMallocDefault));
+    cap = c;
+    return GFC_OK;
+}
+
+int grow_part(gfc_server *s, size_t blocks) {
+    if (blocks <= s->part_cap) return GFC_OK;
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    hipFree(s->d_part);
+    s->d_part = nullptr;
+    HIP_TRY(hipMalloc(&s->d_part, blocks * 3 * sizeof(double)));
+    s->part_cap = blocks;
+    return GFC_OK;
+}
+
+void fill_desc(gfc_server *s, Req &r) {
+    BatchJob &b = s->h_jobs[r.slot];
+    gfc_job *j = r.job;
+    b.u0 = r.fp32 ? (const void *)j->u0f : (const void *)j->u0;
+    b.g = r.fp32 ? (void *)j->gf : (void *)j->g;
+    b.u0d = j->u0;
+    b.gf = j->gf;
+    b.omega = (double)r.p.omega_OR;
+    b.ns = j->geom.ns;
+    b.Vh = j->Vh;
+    b.hits = (int)r.p.hits;
+}
+
+int upload_desc(gfc_server *s) {
+    HIP_TRY(hipMemcpyAsync(s->d_jobs, s->h_jobs, sizeof(BatchJob) * s->max_jobs,
+                           hipMemcpyHostToDevice, s->ctx->stream));
+    return GFC_OK;
+}
+
+// Takes a free slot and a job (from the cache or newly allocated) for r.
+int admit_one(gfc_server *s, Req &r, bool &admitted) {
+    admitted = false;
+    if ((int)s->active.size() >= s->max_jobs) return GFC_OK;
+    auto key = std::make_tuple(r.geom.ns, r.geom.nt, r.prec);
+    auto it = s->cache.find(key);
+    gfc_job *job = nullptr;
+    if (it != s->cache.end()) {
+        job = it->second;
+        s->cache.erase(it);
+    } else {
+        // evict cached jobs of other shapes until the footprint fits
+        while (s->used + r.footprint > s->budget && !s->cache.empty()) {
+            auto e = s->cache.begin();
+            s->used -= gfc_job_footprint(&e->second->geom, e->second->precision);
+            gfc_job_destroy(e->second);
+            s->cache.erase(e);
+        }
+        if (s->used + r.footprint > s->budget) return GFC_OK;  // wait for memory
+        RC_TRY(gfc_job_create(s->ctx, &r.geom, r.prec, &job));
+        s->used += r.footprint;
+    }
+    r.job = job;
+    // slot: lowest index not used by another active request
+    std::vector<char> busy(s->max_jobs, 0);
+    for (auto &a : s->active) busy[a->slot] = 1;
+    int slot = 0;
+    while (busy[slot]) slot++;
+    r.slot = slot;
+    // input: synthesised in HBM, reunitarized like the CLI does after loading
+    RC_TRY(gfcx::generate_smooth_device(job, r.seed, r.eps));
+    hipLaunchKernelGGL(gfc::k_reunit_aos, dim3(nblocks(job->V * 4)), dim3(kBlock), 0,
+                       s->ctx->stream, job->cfg, job->V * 4, s->ctx->flag_dev);
+    LAUNCH_CHECK();
+    RC_TRY(gfcx::build_u0(job));
+    RC_TRY(gfcx::set_identity(job));
+    r.fp32 = r.o.precision == GFC_PREC_MIXED;
+    r.state = INIT;
+    r.t_admit = s->clock();
+    fill_desc(s, r);
+    admitted = true;
+    s->stats.admitted++;
+    return GFC_OK;
+}
+
+// Residues of every request in `which` (one batched launch + one sync).
+int evaluate(gfc_server *s, std::vector<Req *> &which) {
+    if (which.empty()) return GFC_OK;
+    size_t nblk = 0;
+    for (Req *r : which) nblk += nblocks(2 * (int64_t)r->job->Vh);
+    RC_TRY(grow_part(s, nblk));
+    RC_TRY(grow_table(s, s->d_oitems, s->h_oitems, s->oitems_cap, nblk));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));  // pinned staging is reused
+    size_t k = 0;
+    for (size_t q = 0; q < which.size(); q++) {
+        Req *r = which[q];
+        s->h_jobs[r->slot].part_off = (int)k;
+        const int nb = (int)nblocks(2 * (int64_t)r->job->Vh);
+        for (int b = 0; b < nb; b++) s->h_oitems[k++] = make_int4(r->slot, b, r->fp32 ? 1 : 0, 0);
+        s->h_sel[q] = r->slot;
+    }
+    hipStream_t st = s->ctx->stream;
+    RC_TRY(upload_desc(s));
+    HIP_TRY(hipMemcpyAsync(s->d_oitems, s->h_oitems, k * sizeof(int4), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->d_sel, s->h_sel, which.size() * sizeof(int), hipMemcpyHostToDevice,
+                           st));
+    hipLaunchKernelGGL(gfc::k_obs_batch, dim3((unsigned)k), dim3(kBlock), 0, st, s->d_jobs,
+                       s->d_oitems, s->d_part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(gfc::k_reduce_batch, dim3((unsigned)which.size()), dim3(kBlock), 0, st,
+                       s->d_jobs, s->d_sel, s->d_part, s->d_out);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(s->h_out, s->d_out, 3 * which.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t q = 0; q < which.size(); q++) {
+        Req *r = which[q];
+        const double V = (double)r->job->V;
+        r->obs[0] = s->h_out[3 * q] / V;
+        r->obs[1] = s->h_out[3 * q + 1] / (3.0 * V);
+        r->obs[2] = s->h_out[3 * q + 2] / (9.0 * V);
+        r->evals++;
+    }
+    s->stats.residue_batches++;
+    return GFC_OK;
+}
+
+int promote(gfc_server *s, Req &r) {
+    RC_TRY(gfcx::promote_g_only(r.job));
+    r.fp32 = false;
+    fill_desc(s, r);
+    s->plan_sig.clear();
+    return GFC_OK;
+}
+
+// End of gaugefixOverrelaxation for r: hand-off, final reunitarization (unless
+// it stopped on max sweeps or a stalled residue), U = G U0 G^dagger.
+int finish(gfc_server *s, Req &r, int rc) {
+    if (r.fp32) RC_TRY(promote(s, r));
+    if (rc >= 0) RC_TRY(gfcx::reunit_g(r.job, false, 0, 2));
+    RC_TRY(gfcx::materialize(r.job));
+    r.rc = rc >= 0 ? r.sweep : rc;
+    r.state = DONE;
+    return GFC_OK;
+}
+
+int next_multiple(int s, int m) { return (s / m + 1) * m; }
+
+int plan_and_run(gfc_server *s) {
+    // stop points: next reunitarization point, next check, next probe, max
+    std::vector<Req *> run;
+    std::vector<int> steps;
+    int K = 0;
+    for (auto &a : s->active) {
+        if (a->state != RUN) continue;
+        Req &r = *a;
+        const int R = (int)r.p.sweeps_to_reunitarization;
+        int stop = next_multiple(r.sweep, R);
+        if (r.due > r.sweep) stop = std::min(stop, r.due);
+        if ((int64_t)r.p.max_sweeps_to_fix > r.sweep)
+            stop = (int)std::min<int64_t>(stop, (int64_t)r.p.max_sweeps_to_fix);
+        if (r.fp32) stop = std::min(stop, next_multiple(r.sweep, r.o.mixed_probe_every));
+        run.push_back(&r);
+        steps.push_back(stop - r.sweep);
+        K = std::max(K, stop - r.sweep);
+    }
+    if (run.empty()) return GFC_OK;
+    // plan signature: (slot, job, fp32, steps, parity) of every runner
+    std::vector<int64_t> sig;
+    for (size_t q = 0; q < run.size(); q++) {
+        sig.push_back(run[q]->slot);
+        sig.push_back((int64_t)(intptr_t)run[q]->job);
+        sig.push_back(run[q]->fp32);
+        sig.push_back(steps[q]);
+        sig.push_back(run[q]->sweep & 1);
+    }
+    hipStream_t st = s->ctx->stream;
+    if (sig != s->plan_sig || s->graph == nullptr) {
+        // item table: for each step k, the 64-bit then the 32-bit runners
+        std::vector<std::pair<size_t, size_t>> seg;  // per (step, precision): offset, count
+        size_t total = 0;
+        for (int k = 0; k < K; k++)
+            for (int prec = 0; prec < 2; prec++)
+                for (size_t q = 0; q < run.size(); q++)
+                    if (steps[q] > k && (int)run[q]->fp32 == prec)
+                        total += nblocks(run[q]->job->Vh);
+        RC_TRY(grow_table(s, s->d_items, s->h_items, s->items_cap, total));
+        HIP_TRY(hipStreamSynchronize(st));
+        size_t n = 0;
+        for (int k = 0; k < K; k++)
+            for (int prec = 0; prec < 2; prec++) {
+                const size_t off = n;
+                for (size_t q = 0; q < run.size(); q++) {
+                    Req &r = *run[q];
+                    if (steps[q] <= k || (int)r.fp32 != prec) continue;
+                    const int par = (r.sweep + k) & 1;
+                    for (int h0 = 0; h0 < r.job->Vh; h0 += kBlock)
+                        s->h_items[n++] = make_int4(r.slot, h0, par, 0);
+                }
+                seg.emplace_back(off, n - off);
+            }
+        RC_TRY(upload_desc(s));
+        HIP_TRY(hipMemcpyAsync(s->d_items, s->h_items, n * sizeof(int4), hipMemcpyHostToDevice,
+                               st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (s->graph) {
+            hipGraphExecDestroy(s->graph);
+            s->graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (size_t i = 0; i < seg.size(); i++) {
+            const size_t off = seg[i].first, cnt = seg[i].second;
+            if (cnt == 0) continue;
+            const int per = (int)std::max<size_t>(1, cnt / 4096);
+            const unsigned grid = (unsigned)((cnt + per - 1) / per);
+            if (i % 2 == 0)
+                hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3(grid), dim3(kBlock), 0, st,
+                                   s->d_jobs, s->d_items + off, (int)cnt, per);
+            else
+                hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3(grid), dim3(kBlock), 0, st,
+                                   s->d_jobs, s->d_items + off, (int)cnt, per);
+        }
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        if (ce != hipSuccess) return fail(GFC_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
+        ce = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ce != hipSuccess) {
+            s->graph = nullptr;
+            return fail(GFC_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ce));
+        }
+        s->plan_sig = sig;
+        s->stats.graphs_built++;
+    } else {
+        s->stats.graph_replays++;
+    }
+    // at most two chunks in flight: new arrivals are admitted behind at most
+    // one queued chunk
+    hipEvent_t &ev = s->chunk_ev[s->stats.chunks % 2];
+    HIP_TRY(hipEventSynchronize(ev));
+    HIP_TRY(hipGraphLaunch(s->graph, st));
+    HIP_TRY(hipEventRecord(ev, st));
+    s->stats.chunks++;
+    for (size_t q = 0; q < run.size(); q++) {
+        run[q]->sweep += steps[q];
+        if (run[q]->fp32) run[q]->sweeps_fp32 += steps[q];
+        s->stats.site_updates += (int64_t)steps[q] * run[q]->job->Vh;
+    }
+    return GFC_OK;
+}
+
+int complete(gfc_server *s, std::unique_ptr<Req> &r) {
+    gfc_completion c;
+    std::memset(&c, 0, sizeof c);
+    c.id = r->id;
+    c.tag = r->tag;
+    c.status = r->rc;
+    c.sweeps = r->sweep;
+    c.sweeps_fp32 = r->sweeps_fp32;
+    c.residue_initial = r->res0;
+    c.residue_final = r->res;
+    c.F = r->F;
+    c.t_submit = r->t_submit;
+    c.t_admit = r->t_admit;
+    c.t_done = r->t_done;
+    c.ns = r->geom.ns;
+    c.nt = r->geom.nt;
+    s->done.push_back(c);
+    s->stats.completed++;
+    if (s->keep) {
+        s->kept[r->id] = std::move(r);
+    } else {
+        s->cache.emplace(std::make_tuple(r->geom.ns, r->geom.nt, r->prec), r->job);
+        r->job = nullptr;
+    }
+    return GFC_OK;
+}
+
+int poll_once(gfc_server *s) {
+    // 1. admission (FIFO, by footprint and slot)
+    while (!s->queue.empty()) {
+        bool ok = false;
+        RC_TRY(admit_one(s, *s->queue.front(), ok));
+        if (!ok) break;
+        s->active.push_back(std::move(s->queue.front()));
+        s->queue.pop_front();
+    }
+    // 2. residues for starts, checks and probes
+    std::vector<Req *> ev;
+    for (auto &a : s->active) {
+        Req &r = *a;
+        if (r.state == INIT) {
+            ev.push_back(&r);
+        } else if (r.state == RUN) {
+            const bool check = r.sweep == r.due;
+            const bool probe = r.fp32 && !check && r.sweep % r.o.mixed_probe_every == 0;
+            if (check || probe) ev.push_back(&r);
+        }
+    }
+    RC_TRY(evaluate(s, ev));
+    // 3. decisions (the order of gaugefixOverrelaxation's loop body)
+    for (auto &a : s->active) {
+        Req &r = *a;
+        const int which = r.p.gfix_proxy == GFC_PROXY_THETA ? 1 : 0;
+        if (r.state == INIT) {
+            r.res = r.res0 = r.obs[which];
+            r.F = r.obs[2];
+            r.due = gfc_next_check(0, r.res, &r.p);
+            r.state = RUN;
+            if (!(r.res > r.p.tolerance)) RC_TRY(finish(s, r, 0));
+            continue;
+        }
+        if (r.state != RUN || r.sweep == 0) continue;
+        const int s_ = r.sweep;
+        const int R = (int)r.p.sweeps_to_reunitarization;
+        const bool at_stop = s_ % R == 0 || s_ == r.due ||
+                             (int64_t)s_ >= (int64_t)r.p.max_sweeps_to_fix ||
+                             (r.fp32 && s_ % r.o.mixed_probe_every == 0);
+        if (!at_stop) continue;
+        if (r.fp32 && s_ != r.due && s_ % r.o.mixed_probe_every == 0) {
+            if (r.obs[which] <= r.o.mixed_switch) RC_TRY(promote(s, r));
+        }
+        if (s_ == r.due) {
+            const double last = r.res;
+            r.res = r.obs[which];
+            r.F = r.obs[2];
+            if (r.fp32 && r.res <= r.o.mixed_switch) RC_TRY(promote(s, r));
+            if (std::fabs(r.res - last) / last < 10E-16) {
+                RC_TRY(finish(s, r, GFC_ENOTCONVERGED));
+                continue;
+            } else if (r.res <= r.p.tolerance) {
+                RC_TRY(finish(s, r, 0));
+                continue;
+            } else {
+                r.due = gfc_next_check((unsigned)s_, r.res, &r.p);
+            }
+        }
+        if ((unsigned)s_ >= r.p.max_sweeps_to_fix) {
+            RC_TRY(finish(s, r, GFC_ENOTCONVERGED));
+            continue;
+        }
+        if (!((unsigned)s_ % r.p.sweeps_to_reunitarization)) RC_TRY(gfcx::reunit_g(r.job, r.fp32, 0, 2));
+    }
+    // retire finished requests (their results are materialised in HBM)
+    bool any_done = false;
+    for (auto &a : s->active) any_done |= a->state == DONE;
+    if (any_done) {
+        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+        const double t = s->clock();
+        for (auto &a : s->active)
+            if (a->state == DONE) {
+                a->t_done = t;
+                RC_TRY(complete(s, a));
+            }
+        s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                       [](const std::unique_ptr<Req> &a) {
+                                           return !a || a->state == DONE || a->job == nullptr;
+                                       }),
+                        s->active.end());
+        s->plan_sig.clear();
+    }
+    // 4. one chunk of packed half-sweeps
+    return plan_and_run(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_results,
+                      gfc_server **out) {
+    if (ctx == nullptr || out == nullptr || max_jobs <= 0)
+        return fail(GFC_EINVAL, "gfc_server_create: bad argument");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto s = std::make_unique<gfc_server>();
+    s->ctx = ctx;
+    s->max_jobs = max_jobs;
+    s->keep = keep_results;
+    s->t0 = std::chrono::steady_clock::now();
+    if (mem_budget <= 0) {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(hipMemGetInfo(&fr, &tot));
+        mem_budget = (int64_t)(0.85 * (double)fr);
+    }
+    s->budget = mem_budget;
+    s->d_jobs = dev_alloc<BatchJob>(max_jobs);
+    s->d_sel = dev_alloc<int>(max_jobs);
+    s->d_out = dev_alloc<double>(3 * (size_t)max_jobs);
+    if (!s->d_jobs || !s->d_sel || !s->d_out) return fail(GFC_ENOMEM, "gfc_server_create");
+    HIP_TRY(hipHostMalloc(&s->h_jobs, sizeof(BatchJob) * max_jobs, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&s->h_sel, sizeof(int) * max_jobs, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&s->h_out, sizeof(double) * 3 * max_jobs, hipHostMallocDefault));
+    std::memset(s->h_jobs, 0, sizeof(BatchJob) * max_jobs);
+    HIP_TRY(hipEventCreateWithFlags(&s->chunk_ev[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&s->chunk_ev[1], hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(s->chunk_ev[0], ctx->stream));
+    HIP_TRY(hipEventRecord(s->chunk_ev[1], ctx->stream));
+    *out = s.release();
+    return GFC_OK;
+}
+
+void gfc_server_destroy(gfc_server *s) {
+    if (s == nullptr) return;
+    hipSetDevice(s->ctx->device);
+    hipStreamSynchronize(s->ctx->stream);
+    if (s->graph) hipGraphExecDestroy(s->graph);
+    for (hipEvent_t e : s->chunk_ev)
+        if (e) hipEventDestroy(e);
+    for (auto &a : s->active) gfc_job_destroy(a->job);
+    for (auto &kv : s->kept) gfc_job_destroy(kv.second->job);
+    for (auto &kv : s->cache) gfc_job_destroy(kv.second);
+    hipFree(s->d_jobs);
+    hipFree(s->d_items);
+    hipFree(s->d_oitems);
+    hipFree(s->d_sel);
+    hipFree(s->d_part);
+    hipFree(s->d_out);
+    hipHostFree(s->h_jobs);
+    hipHostFree(s->h_items);
+    hipHostFree(s->h_oitems);
+    hipHostFree(s->h_sel);
+    hipHostFree(s->h_out);
+    delete s;
+}
+
+double gfc_server_clock(gfc_server *s) { return s ? s->clock() : 0.0; }
+
+int64_t gfc_server_submit(gfc_server *s, const gfc_geom *g, uint64_t seed, double eps,
+                          const gfc_params *p, const gfc_options *o, uint64_t tag) {
+    if (s == nullptr || g == nullptr || p == nullptr)
+        return fail(GFC_EINVAL, "gfc_server_submit: NULL argument");
+    if (gfc_geom_check(g)) return fail(GFC_EGEOMETRY, "gfc_server_submit: bad geometry");
+    if (!gfc_params_valid(p)) return fail(GFC_EPARAMS, "gfc_server_submit: invalid parameters");
+    auto r = std::make_unique<Req>();
+    r->id = s->next_id++;
+    r->tag = tag;
+    r->geom = *g;
+    r->seed = seed;
+    r->eps = eps;
+    r->p = *p;
+    if (o)
+        r->o = *o;
+    else
+        gfc_options_default(&r->o);
+    if (r->o.mixed_probe_every <= 0) r->o.mixed_probe_every = 50;
+    r->prec = r->o.precision == GFC_PREC_MIXED ? GFC_PREC_MIXED : GFC_PREC_FP64;
+    r->footprint = footprint_of(*r);
+    if (r->footprint > s->budget)
+        return fail(GFC_ENOMEM, "gfc_server_submit: lattice larger than the memory budget");
+    r->t_submit = s->clock();
+    const int64_t id = r->id;
+    s->queue.push_back(std::move(r));
+    s->stats.submitted++;
+    return id;
+}
+
+int gfc_server_poll(gfc_server *s, gfc_completion *out, int max_out) {
+    if (s == nullptr || (out == nullptr && max_out > 0))
+        return fail(GFC_EINVAL, "gfc_server_poll: bad argument");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    if (s->done.empty()) RC_TRY(poll_once(s));
+    const int n = std::min<int>(max_out, (int)s->done.size());
+    for (int i = 0; i < n; i++) out[i] = s->done[i];
+    s->done.erase(s->done.begin(), s->done.begin() + n);
+    return n;
+}
+
+int gfc_server_pending(gfc_server *s) {
+    return s ? (int)(s->queue.size() + s->active.size() + s->done.size()) : 0;
+}
+
+int gfc_server_get_stats(gfc_server *s, gfc_server_stats *st) {
+    if (s == nullptr || st == nullptr) return fail(GFC_EINVAL, "gfc_server_get_stats: NULL");
+    *st = s->stats;
+    st->queued = (int)s->queue.size();
+    st->active = (int)s->active.size();
+    st->bytes_used = s->used;
+    st->bytes_budget = s->budget;
+    return GFC_OK;
+}
+
+int gfc_server_result_gauge(gfc_server *s, int64_t id, double *G) {
+    if (s == nullptr || G == nullptr) return fail(GFC_EINVAL, "gfc_server_result_gauge: NULL");
+    auto it = s->kept.find(id);
+    if (it == s->kept.end()) return fail(GFC_EINVAL, "gfc_server_result_gauge: unknown id");
+    return gfc_job_download_gauge(it->second->job, G);
+}
+
+int gfc_server_release(gfc_server *s, int64_t id) {
+    if (s == nullptr) return fail(GFC_EINVAL, "gfc_server_release: NULL");
+    auto it = s->kept.find(id);
+    if (it == s->kept.end()) return fail(GFC_EINVAL, "gfc_server_release: unknown id");
+    Req &r = *it->second;
+    s->cache.emplace(std::make_tuple(r.geom.ns, r.geom.nt, r.prec), r.job);
+    s->kept.erase(it);
+    return GFC_OK;
+}
+
+}  // extern "C"

@@ -68,6 +68,7 @@ class _Options(C.Structure):
         ("mixed_probe_every", C.c_int),
         ("verbose", C.c_int),
         ("time_sweeps", C.c_int),
+        ("retire_slices", C.c_int),
     ]
 
 
@@ -80,6 +81,35 @@ class _Report(C.Structure):
         ("residue_final", C.c_double),
         ("seconds", C.c_double),
         ("seconds_sweeps", C.c_double),
+        ("slice_sweeps", C.c_double),
+    ]
+
+
+class _Completion(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64),
+        ("tag", C.c_uint64),
+        ("status", C.c_int),
+        ("sweeps", C.c_int),
+        ("sweeps_fp32", C.c_int),
+        ("ns", C.c_int),
+        ("nt", C.c_int),
+        ("residue_initial", C.c_double),
+        ("residue_final", C.c_double),
+        ("F", C.c_double),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+        ("chunks", C.c_int64), ("graphs_built", C.c_int64), ("graph_replays", C.c_int64),
+        ("residue_batches", C.c_int64), ("site_updates", C.c_int64),
+        ("queued", C.c_int), ("active", C.c_int),
+        ("bytes_used", C.c_int64), ("bytes_budget", C.c_int64),
     ]
 
 
@@ -130,27 +160,26 @@ def _load():
                                    C.POINTER(_Options), D, D, C.POINTER(_Report)]),
         "gfc_reunitarize_field": (I, [P, D, I64]),
         "gfc_config_observables": (I, [P, C.POINTER(_Geom), D, D, D, D, D, D]),
+        "gfc_job_generate_smooth": (I, [P, C.c_uint64, C.c_double]),
+        "gfc_server_create": (I, [P, I64, I, I, C.POINTER(P)]),
+        "gfc_server_destroy": (None, [P]),
+        "gfc_server_clock": (C.c_double, [P]),
+        "gfc_server_submit": (I64, [P, C.POINTER(_Geom), C.c_uint64, C.c_double,
+                                    C.POINTER(_Params), C.POINTER(_Options), C.c_uint64]),
+        "gfc_server_poll": (I, [P, C.POINTER(_Completion), I]),
+        "gfc_server_pending": (I, [P]),
+        "gfc_server_get_stats": (I, [P, C.POINTER(_ServerStats)]),
+        "gfc_server_result_gauge": (I, [P, I64, D]),
+        "gfc_server_release": (I, [P, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    return lib
+    return lib, sorted(sig)
 
 
-lib = _load()
-EXPORTED = sorted(
-    ["gfc_params_default", "gfc_params_from_file", "gfc_params_valid", "gfc_params_print",
-     "gfc_next_check", "gfc_geom_check", "gfc_geom_volume", "gfc_load_config",
-     "gfc_write_config", "gfc_write_gauge_transf", "gfc_load_gauge_transf",
-     "gfc_write_sweeps_record", "gfc_generate_smooth", "gfc_field_to_fp32",
-     "gfc_options_default", "gfc_last_error", "gfc_version", "gfc_ctx_create",
-     "gfc_ctx_destroy", "gfc_ctx_copy_probe", "gfc_job_footprint", "gfc_job_create", "gfc_job_destroy",
-     "gfc_job_upload_config", "gfc_job_reunitarize", "gfc_job_observables",
-     "gfc_job_plaquette", "gfc_job_fix", "gfc_job_fix_trace", "gfc_job_download_config",
-     "gfc_job_download_gauge", "gfc_job_get_link", "gfc_job_save_config",
-     "gfc_job_restore_config", "gfc_fix_coulomb_or",
-     "gfc_reunitarize_field", "gfc_config_observables"])
+lib, EXPORTED = _load()
 
 
 def _dptr(a: np.ndarray):
@@ -229,10 +258,11 @@ class Options:
     mixed_probe_every: int = 50
     verbose: int = 0
     time_sweeps: int = 0
+    retire_slices: int = 0
 
     def _c(self) -> _Options:
         return _Options(self.precision, self.mixed_switch, self.mixed_probe_every, self.verbose,
-                        self.time_sweeps)
+                        self.time_sweeps, self.retire_slices)
 
 
 @dataclass
@@ -244,11 +274,12 @@ class Report:
     residue_final: float
     seconds: float
     seconds_sweeps: float
+    slice_sweeps: float
 
 
 def _report(r: _Report) -> Report:
     return Report(r.sweeps, r.sweeps_fp32, r.residue_evals, r.residue_initial,
-                  r.residue_final, r.seconds, r.seconds_sweeps)
+                  r.residue_final, r.seconds, r.seconds_sweeps, r.slice_sweeps)
 
 
 def new_config(geom: Geometry) -> np.ndarray:
@@ -413,6 +444,10 @@ class Job:
         _check(lib.gfc_job_download_gauge(self._p, _dptr(G)))
         return G
 
+    def generate_smooth(self, seed: int, eps: float):
+        """Synthesise the smooth configuration directly in HBM (G <- 1)."""
+        _check(lib.gfc_job_generate_smooth(self._p, seed, eps))
+
     def save_config(self):
         _check(lib.gfc_job_save_config(self._p))
 
@@ -453,3 +488,92 @@ def config_observables(ctx: Context, geom: Geometry, U: np.ndarray):
     _check(lib.gfc_config_observables(ctx._p, C.byref(g), _dptr(U),
                                       *[C.byref(v) for v in vals]))
     return tuple(v.value for v in vals)
+
+
+@dataclass
+class Completion:
+    id: int
+    tag: int
+    status: int
+    sweeps: int
+    sweeps_fp32: int
+    ns: int
+    nt: int
+    residue_initial: float
+    residue_final: float
+    F: float
+    t_submit: float
+    t_admit: float
+    t_done: float
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_submit
+
+
+class Server:
+    """Continuous-batching scheduler (gfc_server_*): many lattices resident in
+    HBM, admitted by footprint, their half-sweeps packed into shared launches."""
+
+    def __init__(self, ctx: Context, mem_budget: int = 0, max_jobs: int = 64,
+                 keep_results: bool = False):
+        self.ctx = ctx
+        self._p = C.c_void_p()
+        _check(lib.gfc_server_create(ctx._p, mem_budget, max_jobs, int(keep_results),
+                                     C.byref(self._p)))
+        self._buf = (_Completion * 64)()
+
+    def close(self):
+        if self._p:
+            lib.gfc_server_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def clock(self) -> float:
+        return lib.gfc_server_clock(self._p)
+
+    def submit(self, geom: Geometry, seed: int, eps: float, params: Params,
+               options: Options | None = None, tag: int = 0) -> int:
+        g, p = geom._c(), params._c()
+        o = (options or Options())._c()
+        return _check(lib.gfc_server_submit(self._p, C.byref(g), seed, eps, C.byref(p),
+                                            C.byref(o), tag))
+
+    def poll(self) -> list:
+        n = _check(lib.gfc_server_poll(self._p, self._buf, len(self._buf)))
+        return [Completion(*(getattr(self._buf[i], f) for f, _ in _Completion._fields_))
+                for i in range(n)]
+
+    def pending(self) -> int:
+        return lib.gfc_server_pending(self._p)
+
+    def stats(self) -> dict:
+        s = _ServerStats()
+        _check(lib.gfc_server_get_stats(self._p, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _ServerStats._fields_}
+
+    def result_gauge(self, rid: int, geom: Geometry) -> np.ndarray:
+        G = np.zeros((geom.volume, 3, 3), dtype=np.complex128)
+        _check(lib.gfc_server_result_gauge(self._p, rid, _dptr(G)))
+        return G
+
+    def release(self, rid: int):
+        _check(lib.gfc_server_release(self._p, rid))
+
+    def drain(self, timeout: float = 600.0) -> list:
+        """Polls until every submitted request has completed."""
+        out = []
+        t0 = self.clock()
+        while self.pending() > 0:
+            out += self.poll()
+            if self.clock() - t0 > timeout:
+                raise TimeoutError("server did not drain")
+        return out

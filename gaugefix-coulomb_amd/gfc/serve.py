@@ -1,0 +1,94 @@
+"""Request stream for the continuous-batching server (gfc.Server).
+
+`stream_requests` draws the serving workload: Poisson arrivals, lattice sizes
+from a weighted set, eps uniform in a range, tolerance from a set; every
+request is an independent seeded smooth configuration.  `run_stream` replays
+the arrival times in real time against a server, polling it between arrivals,
+and measures each request's latency from its scheduled arrival to the moment
+its gauge transformation is materialised in HBM.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import Geometry, Options, Params, PREC_MIXED, Server
+
+CONFIG4_SIZES = ((16, 32, 0.4), (24, 48, 0.3), (32, 64, 0.2), (48, 96, 0.1))
+
+
+@dataclass
+class Request:
+    idx: int
+    t_arrival: float
+    geom: Geometry
+    seed: int
+    eps: float
+    tolerance: float
+
+
+def stream_requests(n=256, rate=4.0, sizes=CONFIG4_SIZES, eps_range=(0.2, 0.4),
+                    tolerances=(1e-14, 1e-16), seed=4, base_seed=1000):
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=n)
+    t = np.cumsum(gaps) - gaps[0]
+    probs = np.array([s[2] for s in sizes])
+    kinds = rng.choice(len(sizes), size=n, p=probs / probs.sum())
+    eps = rng.uniform(*eps_range, size=n)
+    tols = rng.choice(tolerances, size=n)
+    return [Request(i, float(t[i]), Geometry(sizes[kinds[i]][0], sizes[kinds[i]][1]),
+                    base_seed + i, float(eps[i]), float(tols[i])) for i in range(n)]
+
+
+def request_params(r: Request) -> Params:
+    return Params(tolerance=r.tolerance, sweeps_to_reunitarization=10, omega_OR=1.95, hits=2,
+                  estimate_sweeps_to_gf_progress=1000)
+
+
+def run_stream(server: Server, reqs, options: Options | None = None, log=None):
+    """Submits reqs at their arrival times (relative to now) and polls the server
+    until all have completed.  Returns a summary dict and the completions."""
+    options = options or Options(precision=PREC_MIXED)
+    t0 = server.clock()
+    by_tag = {}
+    i = 0
+    n = len(reqs)
+    last_log = 0.0
+    while len(by_tag) < n:
+        now = server.clock() - t0
+        while i < n and reqs[i].t_arrival <= now:
+            r = reqs[i]
+            server.submit(r.geom, r.seed, r.eps, request_params(r), options, tag=r.idx)
+            i += 1
+        if server.pending() == 0:
+            if i < n:
+                time.sleep(min(0.005, max(0.0, reqs[i].t_arrival - now)))
+            continue
+        for c in server.poll():
+            by_tag[c.tag] = c
+        if log and now - last_log > 10.0:
+            last_log = now
+            log(f"serve: t={now:6.1f}s submitted={i} done={len(by_tag)} "
+                f"pending={server.pending()}")
+    comps = [by_tag[k] for k in range(n)]
+    lat = np.array([c.t_done - (t0 + reqs[c.tag].t_arrival) for c in comps])
+    makespan = max(c.t_done for c in comps) - t0
+    upd = sum(c.sweeps * (c.ns ** 3 * c.nt // 2) for c in comps)
+    st = server.stats()
+    return {
+        "requests": n,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3),
+        "max_ms": float(lat.max() * 1e3),
+        "makespan_s": makespan,
+        "throughput_req_per_s": n / makespan,
+        "offered_req_per_s": n / max(reqs[-1].t_arrival, 1e-9),
+        "site_updates_per_s": upd / makespan,
+        "all_converged": all(c.status >= 0 and c.residue_final <= reqs[c.tag].tolerance
+                             for c in comps),
+        "chunks": st["chunks"], "graph_replays": st["graph_replays"],
+        "graphs_built": st["graphs_built"],
+    }, comps

@@ -127,6 +127,14 @@ typedef struct {
     int time_sweeps;        /* 1: bracket every sweep kernel with HIP events on
                                the job's stream and report their total in
                                gfc_report.seconds_sweeps                       */
+    int retire_slices;      /* 1: per-timeslice convergence tracking.  Every
+                               mixed_probe_every sweeps (both precisions) the
+                               proxy of each timeslice is measured; a slice at
+                               or below the tolerance stops being swept (its
+                               proxy is frozen: w, e2 and theta only couple
+                               sites of one timeslice).  The global proxy is
+                               still checked on the reference schedule.  Needs
+                               ns a multiple of 8; not the drop-in default.    */
 } gfc_options;
 
 typedef struct {
@@ -137,6 +145,8 @@ typedef struct {
     double residue_final;   /* proxy at the last evaluation                    */
     double seconds;         /* wall time of the fixing loop on the device      */
     double seconds_sweeps;  /* of which in sweep kernels (HIP events)          */
+    double slice_sweeps;    /* retire_slices: timeslice half-sweeps performed
+                               divided by nt (equals sweeps without retirement) */
 } gfc_report;
 
 void gfc_options_default(gfc_options *o);
@@ -166,6 +176,11 @@ void gfc_job_destroy(gfc_job *job);
 /* Copies a host configuration (4*V matrices; precision 8 = double, 4 = float)
  * to the device; G is set to the identity. */
 int gfc_job_upload_config(gfc_job *job, const void *U, int in_precision);
+/* Synthesises the smooth configuration of gfc_generate_smooth directly in the
+ * job's device memory (same construction; agrees with the host generator to
+ * rounding, not bit for bit); G is set to the identity.  No counterpart in the
+ * reference: it feeds the serving scheduler without host-side generation. */
+int gfc_job_generate_smooth(gfc_job *job, uint64_t seed, double eps);
 /* Projects all 4*V links onto SU(3) (main's initial reunitarizeField,
  * gauge_fix_coulomb.c:126-131).  Returns GFC_EPROJECT if a link had a null
  * row, like reunitarizeField's -1. */
@@ -212,6 +227,56 @@ int gfc_reunitarize_field(gfc_ctx *ctx, double *field, int64_t n);
 int gfc_config_observables(gfc_ctx *ctx, const gfc_geom *g, const double *U, double *e2,
                            double *theta, double *F, double *plaq_spatial,
                            double *plaq_temporal);
+
+/* ---- serving: continuous batching of many lattices (no counterpart in the
+ * reference; each request runs exactly gfc_job_fix's algorithm) ------------- */
+typedef struct gfc_server gfc_server;
+
+typedef struct {
+    int64_t id;
+    uint64_t tag;
+    int status;             /* sweep count, or GFC_ENOTCONVERGED               */
+    int sweeps;             /* half-sweeps performed                           */
+    int sweeps_fp32;        /* of which in FP32                                 */
+    int ns, nt;
+    double residue_initial; /* proxy before the first sweep                     */
+    double residue_final;   /* proxy at the last check                          */
+    double F;               /* gauge functional at the last check               */
+    double t_submit, t_admit, t_done; /* seconds on the server clock            */
+} gfc_completion;
+
+typedef struct {
+    int64_t submitted, admitted, completed;
+    int64_t chunks, graphs_built, graph_replays, residue_batches;
+    int64_t site_updates;
+    int queued, active;
+    int64_t bytes_used, bytes_budget;
+} gfc_server_stats;
+
+/* A scheduler on ctx's stream.  mem_budget (bytes, <= 0: 85% of the free HBM)
+ * bounds the summed footprints of resident lattices; max_jobs bounds their
+ * number; keep_results keeps finished lattices resident until released. */
+int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_results,
+                      gfc_server **server);
+void gfc_server_destroy(gfc_server *server);
+/* Seconds since the server was created (the clock of gfc_completion). */
+double gfc_server_clock(gfc_server *server);
+/* Queues a request: a smooth configuration (gfc_generate_smooth's construction,
+ * synthesised in HBM from seed and eps) to be gauge-fixed with p and o.
+ * Returns the request id (> 0) or an error code. */
+int64_t gfc_server_submit(gfc_server *server, const gfc_geom *g, uint64_t seed, double eps,
+                          const gfc_params *p, const gfc_options *o, uint64_t tag);
+/* One scheduling round: admit what fits, evaluate the residues that are due,
+ * apply the per-lattice decisions, retire finished lattices and launch one
+ * chunk of packed half-sweeps.  Writes up to max_out completions; returns
+ * their number or an error code. */
+int gfc_server_poll(gfc_server *server, gfc_completion *out, int max_out);
+/* Requests queued, resident or completed but not yet returned by poll. */
+int gfc_server_pending(gfc_server *server);
+int gfc_server_get_stats(gfc_server *server, gfc_server_stats *stats);
+/* With keep_results: G (V*18 doubles) of a completed request, and release. */
+int gfc_server_result_gauge(gfc_server *server, int64_t id, double *G);
+int gfc_server_release(gfc_server *server, int64_t id);
 
 #ifdef __cplusplus
 }
