@@ -610,9 +610,14 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
         }
         if (sweep == due) {
             const double last = res;
-            RC_TRY(obs_factored(j, st.fp32, ob));
+            if (sl.on) {
+                // the per-slice measurement also gives the global proxy
+                RC_TRY(sl.probe(j, st.fp32, which, p->tolerance, &res));
+            } else {
+                RC_TRY(obs_factored(j, st.fp32, ob));
+                res = ob[which];
+            }
             r.residue_evals++;
-            res = ob[which];
             if (st.fp32 && res <= o.mixed_switch) RC_TRY(promote_g(j, st));
             if (std::fabs(res - last) / last < 10E-16) {
                 if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", sweep, res);

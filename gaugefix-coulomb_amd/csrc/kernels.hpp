@@ -578,10 +578,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restric
 // bits alone or packed with others.
 
 struct BatchJob {
-    const void *u0;   // cplx<T> planes of the lattice's u0 (T = launch type)
-    void *g;          // cplx<T> planes of its G
+    const void *u0;   // cplx<T> planes of the lattice's u0 (T = sweep launch type)
+    void *g;          // cplx<T> planes of its G (T = sweep launch type)
     const void *u0d;  // FP64 u0 (observables always read FP64 links)
-    const void *gf;   // FP32 G (MIXED), for FP32-state observables
+    const void *gd;   // FP64 G, for FP64-state observables
+    const void *gf;   // FP32 G (MIXED only, else null), for FP32-state observables
     double omega;
     int ns, Vh, hits;
     int part_off;     // first partial-sum slot of this lattice (observables)
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_obs_batch(const BatchJob *__restrict
     const cplx<double> *u0 = static_cast<const cplx<double> *>(J.u0d);
     Obs o = it.z ? obs_site<double, float>(u0, static_cast<const cplx<float> *>(J.gf), J.ns,
                                            J.Vh, idx)
-                 : obs_site<double, double>(u0, static_cast<const cplx<double> *>(J.g), J.ns,
+                 : obs_site<double, double>(u0, static_cast<const cplx<double> *>(J.gd), J.ns,
                                             J.Vh, idx);
     block_reduce3(o, part + 3 * (int64_t)J.part_off, it.y);
 }

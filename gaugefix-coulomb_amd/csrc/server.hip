@@ -138,6 +138,7 @@ void fill_desc(gfc_server *s, Req &r) {
     b.u0 = r.fp32 ? (const void *)j->u0f : (const void *)j->u0;
     b.g = r.fp32 ? (void *)j->gf : (void *)j->g;
     b.u0d = j->u0;
+    b.gd = j->g;
     b.gf = j->gf;
     b.omega = (double)r.p.omega_OR;
     b.ns = j->geom.ns;
@@ -209,7 +210,9 @@ int evaluate(gfc_server *s, std::vector<Req *> &which) {
         Req *r = which[q];
         s->h_jobs[r->slot].part_off = (int)k;
         const int nb = (int)nblocks(2 * (int64_t)r->job->Vh);
-        for (int b = 0; b < nb; b++) s->h_oitems[k++] = make_int4(r->slot, b, r->fp32 ? 1 : 0, 0);
+        // the initial residue reads the FP64 G, like run_fix (engine.hip)
+        const int g32 = r->state == RUN && r->fp32 ? 1 : 0;
+        for (int b = 0; b < nb; b++) s->h_oitems[k++] = make_int4(r->slot, b, g32, 0);
         s->h_sel[q] = r->slot;
     }
     hipStream_t st = s->ctx->stream;
