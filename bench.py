@@ -111,16 +111,17 @@ def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
 
 
 def profile_kernels(ctx, job, n: int):
-    """n FP64 half-sweeps, n FP32 half-sweeps and 1 GiB copies, for counters."""
+    """n FP64 half-sweeps, n FP32 half-sweeps (one whole-lattice launch each)
+    and 1 GiB copies, for counters and per-launch durations."""
     import gfc
     fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=100000000,
                        max_sweeps_to_fix=n, sweeps_to_reunitarization=10 ** 6)
     job.restore_config()
-    rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_FP64))
+    rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_FP64, slice_group=-1))
     log(f"profile: {rep.sweeps} FP64 half-sweeps")
     job.restore_config()
     rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0,
-                                         mixed_probe_every=10 ** 6))
+                                         mixed_probe_every=10 ** 6, slice_group=-1))
     log(f"profile: {rep.sweeps_fp32} FP32 half-sweeps")
     sec = ctx.copy_probe(2 ** 30, reps=3)
     log(f"profile: copied 1 GiB x 4 with k_copy16 (calibration kernel), "
@@ -197,9 +198,11 @@ def main():
     log(f"fp64: {a.steps} steps, sweeps={sweeps}, {step_s:.3f}s/step, {value:.4e} upd/s, "
         f"e2={e2_64:.3e}")
 
-    # sweep-kernel duration with HIP events on the engine stream (untimed run)
+    # sweep-kernel duration with HIP events on the engine stream (untimed run),
+    # one whole-lattice launch per half-sweep
     job.restore_config()
-    rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1))
+    rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1,
+                                          slice_group=-1))
     sweep_ms = rep.seconds_sweeps / rep.sweeps * 1e3
     achieved = BYTES_PER_UPDATE_FP64 * vh / (sweep_ms * 1e-3) / 1e9
     log(f"fp64 sweep kernel: {sweep_ms:.4f} ms/launch, {achieved:.1f} GB/s algorithmic")
@@ -220,12 +223,15 @@ def main():
         mixed_s, rep_m = time_fix(params, mixed, a.steps)
         e2m, thm, Fm = job.observables()
         plaqm = job.plaquette()
+        # hand-off as soon as the FP32 phase is finished (no deferral)
+        imm_s, rep_i = time_fix(params, gfc.Options(precision=gfc.PREC_MIXED, mixed_defer=0), 1)
         # FP32 sweep-kernel duration (HIP events), 50 FP32 half-sweeps
         job.restore_config()
         rc, rep32 = job.fix(gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
                                        max_sweeps_to_fix=50, sweeps_to_reunitarization=10),
                             gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0,
-                                        mixed_probe_every=10 ** 6, time_sweeps=1))
+                                        mixed_probe_every=10 ** 6, time_sweeps=1,
+                                        slice_group=-1))
         sweep32_ms = rep32.seconds_sweeps / rep32.sweeps * 1e3
         # the same comparison under the dense check policy (a check every 10 sweeps)
         dense = gfc.Params(tolerance=1e-16, estimate_sweeps_to_gf_progress=0,
@@ -239,6 +245,9 @@ def main():
             "mixed_final_e2": e2m, "mixed_F": Fm,
             "mixed_F_rel_diff_vs_fp64": abs(Fm - F64) / abs(F64),
             "mixed_plaquette_drift": max(abs(plaqm[0] - plaq0[0]), abs(plaqm[1] - plaq0[1])),
+            "mixed_immediate_handoff": {"seconds": round(imm_s, 4), "sweeps": rep_i.sweeps,
+                                        "sweeps_fp32": rep_i.sweeps_fp32,
+                                        "speedup_vs_fp64": round(step_s / imm_s, 3)},
             "fp32_sweep_kernel_ms": round(sweep32_ms, 5),
             "fp32_sweep_GBs_algorithmic": round(BYTES_PER_UPDATE_FP32 * vh / sweep32_ms / 1e6, 1),
             "dense_schedule": {"fp64_seconds": round(d64_s, 4), "fp64_sweeps": d64.sweeps,
@@ -247,7 +256,8 @@ def main():
                                "mixed_speedup_vs_fp64": round(d64_s / dmx_s, 3)},
         }
         log(f"mixed: {mixed_s:.3f}s sweeps={rep_m.sweeps} fp32={rep_m.sweeps_fp32} e2={e2m:.3e} "
-            f"speedup={step_s / mixed_s:.3f}; fp32 sweep {sweep32_ms:.4f} ms; dense schedule: "
+            f"speedup={step_s / mixed_s:.3f}; immediate hand-off {imm_s:.3f}s "
+            f"(fp32 {rep_i.sweeps_fp32}) speedup {step_s / imm_s:.3f}; fp32 sweep {sweep32_ms:.4f} ms; dense schedule: "
             f"fp64 {d64_s:.3f}s ({d64.sweeps}) mixed {dmx_s:.3f}s ({dmx.sweeps}, "
             f"fp32 {dmx.sweeps_fp32}) speedup {d64_s / dmx_s:.3f}")
     job.close()

@@ -8,6 +8,7 @@
 // sweeps_to_reunitarization sweeps and once at the end.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -36,9 +37,11 @@ void gfc_options_default(gfc_options *o) {
     o->precision = GFC_PREC_FP64;
     o->mixed_switch = 1e-9;
     o->mixed_probe_every = 50;
+    o->mixed_defer = 1;
     o->verbose = 0;
     o->time_sweeps = 0;
     o->retire_slices = 0;
+    o->slice_group = 0;
 }
 
 int gfc_ctx_create(int device, gfc_ctx **out) {
@@ -55,6 +58,12 @@ int gfc_ctx_create(int device, gfc_ctx **out) {
     if (e == hipSuccess) e = hipMalloc(&c->res_dev, 4 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc(&c->res_host, 4 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&c->flag_dev, sizeof(int));
+    c->lane[0] = c->stream;
+    for (int k = 1; k < gfc_ctx::kLanes && e == hipSuccess; k++)
+        e = hipStreamCreateWithFlags(&c->lane[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    for (int k = 0; k < gfc_ctx::kLanes && e == hipSuccess; k++)
+        e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
     if (e != hipSuccess) {
         gfc_ctx_destroy(c);
         return fail(GFC_EDEVICE, std::string("gfc_ctx_create: ") + hipGetErrorString(e));
@@ -71,6 +80,14 @@ void gfc_ctx_destroy(gfc_ctx *c) {
     hipFree(c->res_dev);
     hipHostFree(c->res_host);
     hipFree(c->flag_dev);
+    for (int k = 1; k < gfc_ctx::kLanes; k++)
+        if (c->lane[k]) {
+            hipStreamSynchronize(c->lane[k]);
+            hipStreamDestroy(c->lane[k]);
+        }
+    if (c->fork) hipEventDestroy(c->fork);
+    for (int k = 0; k < gfc_ctx::kLanes; k++)
+        if (c->join[k]) hipEventDestroy(c->join[k]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -113,11 +130,13 @@ int gfc_ctx_copy_probe(gfc_ctx *c, int64_t bytes, int reps, double *seconds) {
 int64_t gfc_job_footprint(const gfc_geom *g, int precision) {
     if (gfc_geom_check(g)) return -1;
     const int64_t V = gfc_geom_volume(g);
-    // u0 and G keep two rows (6 complex numbers) per SU(3) matrix
-    int64_t b = V * 4 * 144      // cfg
-                + V * 3 * 96     // u0
-                + V * 96;        // g
-    if (precision == GFC_PREC_MIXED) b += V * 3 * 48 + V * 48;
+    // u0 and G keep two rows (6 complex numbers) per SU(3) matrix, in fields
+    // of whole 32-site tiles per parity and direction
+    const int64_t fs = gfc::field_size(V / 2);
+    int64_t b = V * 4 * 144       // cfg
+                + 6 * fs * 16     // u0
+                + 2 * fs * 16;    // g
+    if (precision == GFC_PREC_MIXED) b += 6 * fs * 8 + 2 * fs * 8;
     return b;
 }
 
@@ -171,8 +190,10 @@ extern "C" int gfc_job_create(gfc_ctx *ctx, const gfc_geom *g, int precision, gf
     if (precision != GFC_PREC_FP64 && precision != GFC_PREC_MIXED)
         return fail(GFC_EINVAL, "gfc_job_create: bad precision");
     const int64_t V = gfc_geom_volume(g);
-    if (V / 2 > (int64_t)0x3fffffff)
-        return fail(GFC_EGEOMETRY, "gfc_job_create: lattice too large for 32-bit site indices");
+    // the kernels address one parity's field of one direction with 32-bit
+    // byte offsets
+    if (gfc::field_size(V / 2) * (int64_t)sizeof(cplx<double>) > (int64_t)0xffffffffll)
+        return fail(GFC_EGEOMETRY, "gfc_job_create: lattice too large (more than 89478464 sites)");
     HIP_TRY(hipSetDevice(ctx->device));
     gfc_job *j = new gfc_job();
     j->ctx = ctx;
@@ -181,12 +202,12 @@ extern "C" int gfc_job_create(gfc_ctx *ctx, const gfc_geom *g, int precision, gf
     j->Vh = (int)(V / 2);
     j->precision = precision;
     hipError_t e = hipMalloc(&j->cfg, (size_t)V * 4 * 9 * sizeof(cplx<double>));
-    const size_t np = gfc::kPlanes;
-    if (e == hipSuccess) e = hipMalloc(&j->u0, (size_t)V * 3 * np * sizeof(cplx<double>));
-    if (e == hipSuccess) e = hipMalloc(&j->g, (size_t)V * np * sizeof(cplx<double>));
+    const size_t fs = (size_t)gfc::field_size(j->Vh);  // per parity and direction
+    if (e == hipSuccess) e = hipMalloc(&j->u0, 6 * fs * sizeof(cplx<double>));
+    if (e == hipSuccess) e = hipMalloc(&j->g, 2 * fs * sizeof(cplx<double>));
     if (e == hipSuccess && precision == GFC_PREC_MIXED) {
-        e = hipMalloc(&j->u0f, (size_t)V * 3 * np * sizeof(cplx<float>));
-        if (e == hipSuccess) e = hipMalloc(&j->gf, (size_t)V * np * sizeof(cplx<float>));
+        e = hipMalloc(&j->u0f, 6 * fs * sizeof(cplx<float>));
+        if (e == hipSuccess) e = hipMalloc(&j->gf, 2 * fs * sizeof(cplx<float>));
     }
     if (e != hipSuccess) {
         gfc_job_destroy(j);
@@ -363,17 +384,93 @@ int half_sweep(gfc_job *j, SweepState &st, int parity, double omega, int hits) {
     return GFC_OK;
 }
 
+// Projects G on parity indices [h0, h0 + count) of both parities of the live
+// array.
+int reunit_range(gfc_job *j, hipStream_t s, bool fp32, int h0, int count) {
+    const unsigned b = nblocks(2 * (int64_t)count);
+    if (fp32)
+        hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh, 0,
+                           2, h0, count);
+    else
+        hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh, 0,
+                           2, h0, count);
+    LAUNCH_CHECK();
+    return GFC_OK;
+}
+
 // Projects G on parities [p0, p0 + np) of the live array.
 int reunit_g(gfc_job *j, bool fp32, int p0, int np) {
     hipStream_t s = j->ctx->stream;
     const unsigned b = nblocks((int64_t)np * j->Vh);
     if (fp32)
         hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh, p0,
-                           np);
+                           np, 0, j->Vh);
     else
         hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh, p0,
-                           np);
+                           np, 0, j->Vh);
     LAUNCH_CHECK();
+    return GFC_OK;
+}
+
+// Timeslices per group, or 0 for whole-lattice sweeps.  The groups are
+// spread over the context's streams so that one group's kernel tail overlaps
+// the others' work; by default (slice_group = 0) a lattice is cut into about
+// 16 groups, none below 64K parity sites (256 workgroups) per launch, and
+// small lattices are swept whole.
+int group_size(const gfc_job *j, const gfc_options &o) {
+    if (o.slice_group < 0) return 0;
+    const int nt = j->geom.nt;
+    if (o.slice_group > 0) return o.slice_group < nt ? o.slice_group : nt;
+    const int64_t S = (int64_t)j->geom.ns * j->geom.ns * j->geom.ns / 2;
+    int64_t n = std::max<int64_t>((nt + 15) / 16, (65536 + S - 1) / S);
+    return 2 * n <= nt ? (int)n : 0;
+}
+
+// Half-sweeps [from, to) (parity = sweep number & 1), timeslice group by
+// timeslice group, with the reunitarization of G after every sweep number
+// s + 1 < to that is a multiple of R.  Every site update reads only sites of
+// its own timeslice, so each group runs through the sweeps on its own and the
+// result equals that of whole-lattice sweeps bit for bit.
+int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group, double omega,
+                  int hits, unsigned R) {
+    gfc_ctx *c = j->ctx;
+    const int ns = j->geom.ns, nt = j->geom.nt, Vh = j->Vh;
+    const int S = ns * ns * ns / 2;  // parity sites per timeslice
+    const int ngroups = (nt + group - 1) / group;
+    const int lanes = std::min(gfc_ctx::kLanes, ngroups);
+    if (lanes > 1) {
+        HIP_TRY(hipEventRecord(c->fork, c->stream));
+        for (int l = 1; l < lanes; l++) HIP_TRY(hipStreamWaitEvent(c->lane[l], c->fork, 0));
+    }
+    // `lanes` groups at a time, one per stream, launched in step so that the
+    // streams stay busy together
+    for (int g0 = 0; g0 < ngroups; g0 += lanes) {
+        const int g1 = std::min(ngroups, g0 + lanes);
+        for (int k = from; k < to; k++) {
+            for (int gi = g0; gi < g1; gi++) {
+                hipStream_t s = c->lane[gi - g0];
+                const int t0 = gi * group;
+                const int h0 = t0 * S;
+                const int count = (t0 + group <= nt ? group : nt - t0) * S;
+                const unsigned b = nblocks(count);
+                if (st.fp32)
+                    hipLaunchKernelGGL(gfc::k_sweep<float>, dim3(b), dim3(kBlock), 0, s, j->u0f,
+                                       j->gf, ns, Vh, k & 1, (float)omega, hits, h0, count);
+                else
+                    hipLaunchKernelGGL(gfc::k_sweep<double>, dim3(b), dim3(kBlock), 0, s, j->u0,
+                                       j->g, ns, Vh, k & 1, omega, hits, h0, count);
+                LAUNCH_CHECK();
+                if (k + 1 < to && (unsigned)(k + 1) % R == 0)
+                    RC_TRY(reunit_range(j, s, st.fp32, h0, count));
+            }
+        }
+    }
+    if (lanes > 1) {
+        for (int l = 1; l < lanes; l++) {
+            HIP_TRY(hipEventRecord(c->join[l], c->lane[l]));
+            HIP_TRY(hipStreamWaitEvent(c->stream, c->join[l], 0));
+        }
+    }
     return GFC_OK;
 }
 
@@ -580,24 +677,47 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     r.residue_initial = res;
     if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", 0, res);
     int due = gfc_next_check(0, res, p);
+    Handoff ho;
+    ho.start(res);
     int sweep = 0;
     int rc = 0;
     bool done = false;
+    // per-sweep traces and slice retirement need the state after every sweep
+    // of the whole lattice; the other sweeps run timeslice-blocked
+    const bool may_block = trace == nullptr && !sl.on;
     while (res > p->tolerance) {
-        timer.before();
+        const int group = may_block ? group_size(j, o) : 0;
         const bool in32 = st.fp32;
-        if (sl.on)
-            RC_TRY(sl.sweep(j, st.fp32, sweep & 1, omega, hits));
-        else
-            RC_TRY(half_sweep(j, st, sweep & 1, omega, hits));
+        timer.before();
+        if (group > 0) {
+            // the next sweep after which the host looks at the state
+            int64_t stop = due;
+            if (st.fp32) {
+                const int64_t pe = o.mixed_probe_every;
+                if (!ho.at_floor) stop = std::min<int64_t>(stop, (sweep / pe + 1) * pe);
+                if (ho.switch_at > sweep) stop = std::min<int64_t>(stop, ho.switch_at);
+            }
+            stop = std::min<int64_t>(stop, (int64_t)p->max_sweeps_to_fix);
+            if (stop <= sweep) stop = sweep + 1;
+            RC_TRY(sweep_blocked(j, st, sweep, (int)stop, group, omega, hits,
+                                 p->sweeps_to_reunitarization));
+            if (in32) r.sweeps_fp32 += (int)stop - sweep;
+            sweep = (int)stop;
+        } else {
+            if (sl.on)
+                RC_TRY(sl.sweep(j, st.fp32, sweep & 1, omega, hits));
+            else
+                RC_TRY(half_sweep(j, st, sweep & 1, omega, hits));
+            sweep++;
+            if (in32) r.sweeps_fp32++;
+        }
         timer.after();
-        sweep++;
-        if (in32) r.sweeps_fp32++;
         if (trace && sweep <= trace_len) {
             RC_TRY(obs_factored(j, st.fp32, ob));
             trace[sweep - 1] = ob[which];
         }
-        if ((st.fp32 || sl.on) && sweep != due && sweep % o.mixed_probe_every == 0) {
+        if (((st.fp32 && !ho.at_floor) || sl.on) && sweep != due &&
+            sweep % o.mixed_probe_every == 0) {
             double g = 0;
             if (sl.on) {
                 RC_TRY(sl.probe(j, st.fp32, which, p->tolerance, &g));
@@ -606,7 +726,7 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
                 g = ob[which];
             }
             r.residue_evals++;
-            if (st.fp32 && g <= o.mixed_switch) RC_TRY(promote_g(j, st));
+            if (st.fp32 && ho.observe(o, p->tolerance, sweep, g, due)) RC_TRY(promote_g(j, st));
         }
         if (sweep == due) {
             const double last = res;
@@ -618,7 +738,6 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
                 res = ob[which];
             }
             r.residue_evals++;
-            if (st.fp32 && res <= o.mixed_switch) RC_TRY(promote_g(j, st));
             if (std::fabs(res - last) / last < 10E-16) {
                 if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", sweep, res);
                 rc = GFC_ENOTCONVERGED;
@@ -629,8 +748,11 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
             } else {
                 if (o.verbose) printf("Sweeps: %8d.\t residue: %3.2E \n", sweep, res);
                 due = gfc_next_check((unsigned)sweep, res, p);
+                if (st.fp32 && ho.observe(o, p->tolerance, sweep, res, due))
+                    RC_TRY(promote_g(j, st));
             }
         }
+        if (st.fp32 && sweep >= ho.switch_at) RC_TRY(promote_g(j, st));
         if ((unsigned)sweep >= p->max_sweeps_to_fix) {
             rc = GFC_ENOTCONVERGED;
             done = true;

@@ -63,25 +63,76 @@ __device__ __forceinline__ void st_stream(cplx<float> *p, const cplx<float> &c) 
     __builtin_nontemporal_store(v, reinterpret_cast<fvec2 *>(p));
 }
 
+// Tiled layout of a matrix field over n parity sites: the six stored complex
+// numbers of 32 consecutive sites form one 6 x 32 tile, element k of site h at
+// tile_elem(h) + 32 k.  One 32-bit offset (on a uniform base) plus immediate
+// steps of 32 elements addresses a whole matrix, and 32 lanes with
+// consecutive h read 32 consecutive numbers.  A field holds field_size(n)
+// complex numbers (n rounded up to whole tiles) and must stay below 4 GiB.
+constexpr int kTile = 32;
+
+__host__ __device__ constexpr int64_t field_size(int64_t n) {
+    return (n + kTile - 1) / kTile * kTile * kPlanes;
+}
+
+__device__ __forceinline__ uint32_t tile_elem(int h) {
+    return (uint32_t)(h >> 5) * (uint32_t)(kTile * kPlanes) + (uint32_t)(h & (kTile - 1));
+}
+
+template <typename T>
+__device__ __forceinline__ const cplx<T> *elem_ptr(const cplx<T> *field, uint32_t byte_off,
+                                                   int k) {
+    return reinterpret_cast<const cplx<T> *>(reinterpret_cast<const char *>(field) +
+                                             (uint64_t)byte_off + k * kTile * sizeof(cplx<T>));
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> *elem_ptr(cplx<T> *field, uint32_t byte_off, int k) {
+    return reinterpret_cast<cplx<T> *>(reinterpret_cast<char *>(field) + (uint64_t)byte_off +
+                                       k * kTile * sizeof(cplx<T>));
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t tile_bytes(int h) {
+    return tile_elem(h) * (uint32_t)sizeof(cplx<T>);
+}
+
+// Field of the spatial links u0_i on parity p; field of G on parity p.
+template <typename T>
+__device__ __forceinline__ const cplx<T> *u0_field(const cplx<T> *u0, int Vh, int p, int i) {
+    return u0 + (p * 3 + i) * field_size(Vh);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> *u0_field(cplx<T> *u0, int Vh, int p, int i) {
+    return u0 + (p * 3 + i) * field_size(Vh);
+}
+template <typename T>
+__device__ __forceinline__ const cplx<T> *g_field(const cplx<T> *g, int Vh, int p) {
+    return g + p * field_size(Vh);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> *g_field(cplx<T> *g, int Vh, int p) {
+    return g + p * field_size(Vh);
+}
+
 // SU(3) matrix from its two stored rows.
 template <bool STREAM, typename T>
-__device__ __forceinline__ void load_su3(mat3<T> &m, const cplx<T> *__restrict__ plane0,
-                                         int64_t stride, int idx) {
+__device__ __forceinline__ void load_su3(mat3<T> &m, const cplx<T> *__restrict__ field, int h) {
+    const uint32_t off = tile_bytes<T>(h);
 #pragma unroll
     for (int k = 0; k < kPlanes; k++)
-        m.e[k] = STREAM ? ld_stream(plane0 + k * stride + idx) : plane0[k * stride + idx];
+        m.e[k] = STREAM ? ld_stream(elem_ptr(field, off, k)) : *elem_ptr(field, off, k);
     m_fill_row2(m);
 }
 
 template <bool STREAM, typename T>
-__device__ __forceinline__ void store_su3(const mat3<T> &m, cplx<T> *__restrict__ plane0,
-                                          int64_t stride, int idx) {
+__device__ __forceinline__ void store_su3(const mat3<T> &m, cplx<T> *__restrict__ field, int h) {
+    const uint32_t off = tile_bytes<T>(h);
 #pragma unroll
     for (int k = 0; k < kPlanes; k++) {
         if (STREAM)
-            st_stream(plane0 + k * stride + idx, m.e[k]);
+            st_stream(elem_ptr(field, off, k), m.e[k]);
         else
-            plane0[k * stride + idx] = m.e[k];
+            *elem_ptr(field, off, k) = m.e[k];
     }
 }
 
@@ -132,27 +183,26 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
                                            T omega, int hits, int h) {
     const Site s = site_of(h, p, ns);
     const Nbr nb = neighbours(s, h, ns);
-    const int64_t st = Vh;
-    const cplx<T> *gq = g + (int64_t)(1 - p) * kPlanes * st;  // other parity's G
+    const cplx<T> *gq = g_field(static_cast<const cplx<T> *>(g), Vh, 1 - p);  // other parity
     mat3<T> W;
     m_zero(W);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         mat3<T> uf, gf, ub, gb;
-        load_su3<true>(uf, u0 + (int64_t)(p * 3 + i) * kPlanes * st, st, h);
-        load_su3<false>(gf, gq, st, nb.fwd[i]);
-        load_su3<true>(ub, u0 + (int64_t)((1 - p) * 3 + i) * kPlanes * st, st, nb.bwd[i]);
-        load_su3<false>(gb, gq, st, nb.bwd[i]);
+        load_su3<true>(uf, u0_field(u0, Vh, p, i), h);
+        load_su3<false>(gf, gq, nb.fwd[i]);
+        load_su3<true>(ub, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
+        load_su3<false>(gb, gq, nb.bwd[i]);
         m_mac_nd(W, uf, gf);  // u0_i(n) G(n+i)^dagger
         m_mac_dd(W, ub, gb);  // u0_i(n-i)^dagger G(n-i)^dagger
     }
-    cplx<T> *gp = g + (int64_t)p * kPlanes * st;
+    cplx<T> *gp = g_field(g, Vh, p);
     mat3<T> gn, w, upd, out;
-    load_su3<true>(gn, gp, st, h);
+    load_su3<true>(gn, static_cast<const cplx<T> *>(gp), h);
     m_mul(gn, W, w);
     la_or_update(w, hits, omega, upd);
     m_mul(upd, gn, out);
-    store_su3<true>(out, gp, st, h);
+    store_su3<true>(out, gp, h);
 }
 
 // One red/black half-sweep on parity p over parity indices [h0, h0 + count).
@@ -178,42 +228,45 @@ __global__ __launch_bounds__(kBlock) void k_g_identity(cplx<T> *__restrict__ g, 
     const int p = idx >= Vh, h = idx - p * Vh;
     mat3<T> m;
     m_identity(m);
-    store_su3<false>(m, g + (int64_t)p * kPlanes * Vh, Vh, h);
+    store_su3<false>(m, g_field(g, Vh, p), h);
 }
 
 // Projects G onto SU(3) on parities [p0, p0 + np) (reunitarizeField on G,
 // src/fields.c:144-187).  The projection reads rows 0 and 1 only.
 template <typename T>
 __device__ __forceinline__ void reunit_site(cplx<T> *__restrict__ g, int Vh, int p, int h) {
-    cplx<T> *base = g + (int64_t)p * kPlanes * Vh;
+    cplx<T> *base = g_field(g, Vh, p);
+    const uint32_t off = tile_bytes<T>(h);
     mat3<T> m;
 #pragma unroll
-    for (int k = 0; k < kPlanes; k++) m.e[k] = base[(int64_t)k * Vh + h];
+    for (int k = 0; k < kPlanes; k++) m.e[k] = *elem_ptr(base, off, k);
     m_project_su3(m);
-    store_su3<false>(m, base, Vh, h);
+    store_su3<false>(m, base, h);
 }
 
+// Parity indices [h0, h0 + count) of parities [p0, p0 + np).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_g_reunit(cplx<T> *__restrict__ g, int Vh, int p0,
-                                                     int np) {
+                                                     int np, int h0, int count) {
     const int idx = blockIdx.x * kBlock + threadIdx.x;
-    if (idx >= np * Vh) return;
-    reunit_site<T>(g, Vh, p0 + idx / Vh, idx % Vh);
+    if (idx >= np * count) return;
+    reunit_site<T>(g, Vh, p0 + idx / count, h0 + idx % count);
 }
 
 // FP32 -> FP64 hand-off of G: promote rows 0, 1 then project in FP64.
 __device__ __forceinline__ void promote_site(const cplx<float> *__restrict__ gf,
                                              cplx<double> *__restrict__ g, int Vh, int p,
                                              int h) {
-    const cplx<float> *src = gf + (int64_t)p * kPlanes * Vh;
+    const cplx<float> *src = g_field(gf, Vh, p);
+    const uint32_t off = tile_bytes<float>(h);
     mat3<double> m;
 #pragma unroll
     for (int k = 0; k < kPlanes; k++) {
-        cplx<float> v = src[(int64_t)k * Vh + h];
+        cplx<float> v = *elem_ptr(src, off, k);
         m.e[k] = c_make<double>(v.re, v.im);
     }
     m_project_su3(m);
-    store_su3<false>(m, g + (int64_t)p * kPlanes * Vh, Vh, h);
+    store_su3<false>(m, g_field(g, Vh, p), h);
 }
 
 __global__ __launch_bounds__(kBlock) void k_g_promote(const cplx<float> *__restrict__ gf,
@@ -234,15 +287,15 @@ __global__ __launch_bounds__(kBlock) void k_build_u0(const cplx<double> *__restr
     const int p = idx >= Vh, h = idx - p * Vh;
     const Site s = site_of(h, p, ns);
     const int64_t lex = (int64_t)2 * s.row + s.x;
-    const int64_t st = Vh;
+    const uint32_t off = tile_bytes<T>(h);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const cplx<double> *src = cfg + (lex * 4 + 1 + i) * 9;
-        cplx<T> *dst = u0 + (int64_t)(p * 3 + i) * kPlanes * st;
+        cplx<T> *dst = u0_field(u0, Vh, p, i);
 #pragma unroll
         for (int k = 0; k < kPlanes; k++) {
             cplx<double> v = src[k];
-            dst[k * st + h] = c_make<T>((T)v.re, (T)v.im);
+            *elem_ptr(dst, off, k) = c_make<T>((T)v.re, (T)v.im);
         }
     }
 }
@@ -261,10 +314,9 @@ __global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict
     const int z = (int)(r % ns);
     const int t = (int)(r / ns);
     const int p = (x + y + z + t) & 1;
-    const int64_t st = Vh;
     mat3<double> gn;
-    load_su3<false>(gn, g + (int64_t)p * kPlanes * st, st, (int)(lex >> 1));
-    const cplx<double> *gq = g + (int64_t)(1 - p) * kPlanes * st;
+    load_su3<false>(gn, g_field(g, Vh, p), (int)(lex >> 1));
+    const cplx<double> *gq = g_field(g, Vh, 1 - p);
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
         int c[4] = {t, z, y, x};
@@ -272,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict
         if (c[mu] == (mu == 0 ? nt : ns)) c[mu] = 0;
         const int64_t ln = (((int64_t)c[0] * ns + c[1]) * ns + c[2]) * ns + c[3];
         mat3<double> gm, u, tmp, out;
-        load_su3<false>(gm, gq, st, (int)(ln >> 1));
+        load_su3<false>(gm, gq, (int)(ln >> 1));
         cplx<double> *ptr = cfg + (lex * 4 + mu) * 9;
 #pragma unroll
         for (int k = 0; k < 9; k++) u.e[k] = ptr[k];
@@ -318,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_g_to_aos(const cplx<double> *__restr
     const int t = (int)(r / ns);
     const int p = (x + y + z + t) & 1;
     mat3<double> m;
-    load_su3<false>(m, g + (int64_t)p * kPlanes * Vh, Vh, (int)(lex >> 1));
+    load_su3<false>(m, g_field(g, Vh, p), (int)(lex >> 1));
 #pragma unroll
     for (int k = 0; k < 9; k++) out[lex * 9 + k] = m.e[k];
 }
@@ -376,11 +428,12 @@ __device__ __forceinline__ Obs site_obs(mat3<double> &D, double f) {
 }
 
 template <typename T>
-__device__ __forceinline__ void load_su3_d(mat3<double> &m, const cplx<T> *__restrict__ plane0,
-                                           int64_t stride, int idx) {
+__device__ __forceinline__ void load_su3_d(mat3<double> &m, const cplx<T> *__restrict__ field,
+                                           int h) {
+    const uint32_t off = tile_bytes<T>(h);
 #pragma unroll
     for (int k = 0; k < kPlanes; k++) {
-        const cplx<T> v = plane0[k * stride + idx];
+        const cplx<T> v = *elem_ptr(field, off, k);
         m.e[k] = c_make<double>((double)v.re, (double)v.im);
     }
     m_fill_row2(m);
@@ -421,24 +474,23 @@ __device__ __forceinline__ Obs obs_site(const cplx<TU> *__restrict__ u0,
         const int p = idx >= Vh, h = idx - p * Vh;
         const Site s = site_of(h, p, ns);
         const Nbr nb = neighbours(s, h, ns);
-        const int64_t st = Vh;
-        const cplx<TG> *gq = g + (int64_t)(1 - p) * kPlanes * st;
+        const cplx<TG> *gq = g_field(g, Vh, 1 - p);
         mat3<double> gn;
-        load_su3_d(gn, g + (int64_t)p * kPlanes * st, st, h);
+        load_su3_d(gn, g_field(g, Vh, p), h);
         mat3<double> D;
         m_zero(D);
         double f = 0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             mat3<double> u, gm, tmp, lk;
-            load_su3_d(u, u0 + (int64_t)(p * 3 + i) * kPlanes * st, st, h);
-            load_su3_d(gm, gq, st, nb.fwd[i]);
+            load_su3_d(u, u0_field(u0, Vh, p, i), h);
+            load_su3_d(gm, gq, nb.fwd[i]);
             m_mul(gn, u, tmp);
             m_mul_nd(tmp, gm, lk);  // U_i(n)
             accum_link(D, lk, 1.0);
             f += lk.e[0].re + lk.e[4].re + lk.e[8].re;
-            load_su3_d(u, u0 + (int64_t)((1 - p) * 3 + i) * kPlanes * st, st, nb.bwd[i]);
-            load_su3_d(gm, gq, st, nb.bwd[i]);
+            load_su3_d(u, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
+            load_su3_d(gm, gq, nb.bwd[i]);
             m_mul(gm, u, tmp);
             m_mul_nd(tmp, gn, lk);  // U_i(n-i)
             accum_link(D, lk, -1.0);

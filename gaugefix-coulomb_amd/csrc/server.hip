@@ -58,6 +58,7 @@ struct Req {
     int sweep = 0, due = 0, evals = 0, sweeps_fp32 = 0, rc = 0;
     double res = 0, res0 = 0, F = 0;
     bool fp32 = false, finished_early = false;
+    gfcx::Handoff ho;
     int slot = -1;
     double obs[3] = {0, 0, 0};
 };
@@ -275,7 +276,10 @@ int plan_and_run(gfc_server *s) {
         if (r.due > r.sweep) stop = std::min(stop, r.due);
         if ((int64_t)r.p.max_sweeps_to_fix > r.sweep)
             stop = (int)std::min<int64_t>(stop, (int64_t)r.p.max_sweeps_to_fix);
-        if (r.fp32) stop = std::min(stop, next_multiple(r.sweep, r.o.mixed_probe_every));
+        if (r.fp32) {
+            if (!r.ho.at_floor) stop = std::min(stop, next_multiple(r.sweep, r.o.mixed_probe_every));
+            if (r.ho.switch_at > r.sweep) stop = std::min(stop, r.ho.switch_at);
+        }
         run.push_back(&r);
         steps.push_back(stop - r.sweep);
         K = std::max(K, stop - r.sweep);
@@ -409,7 +413,8 @@ int poll_once(gfc_server *s) {
             ev.push_back(&r);
         } else if (r.state == RUN) {
             const bool check = r.sweep == r.due;
-            const bool probe = r.fp32 && !check && r.sweep % r.o.mixed_probe_every == 0;
+            const bool probe = r.fp32 && !r.ho.at_floor && !check &&
+                               r.sweep % r.o.mixed_probe_every == 0;
             if (check || probe) ev.push_back(&r);
         }
     }
@@ -422,6 +427,7 @@ int poll_once(gfc_server *s) {
             r.res = r.res0 = r.obs[which];
             r.F = r.obs[2];
             r.due = gfc_next_check(0, r.res, &r.p);
+            r.ho.start(r.res);
             r.state = RUN;
             if (!(r.res > r.p.tolerance)) RC_TRY(finish(s, r, 0));
             continue;
@@ -431,16 +437,16 @@ int poll_once(gfc_server *s) {
         const int R = (int)r.p.sweeps_to_reunitarization;
         const bool at_stop = s_ % R == 0 || s_ == r.due ||
                              (int64_t)s_ >= (int64_t)r.p.max_sweeps_to_fix ||
-                             (r.fp32 && s_ % r.o.mixed_probe_every == 0);
+                             (r.fp32 && ((!r.ho.at_floor && s_ % r.o.mixed_probe_every == 0) ||
+                                         s_ >= r.ho.switch_at));
         if (!at_stop) continue;
-        if (r.fp32 && s_ != r.due && s_ % r.o.mixed_probe_every == 0) {
-            if (r.obs[which] <= r.o.mixed_switch) RC_TRY(promote(s, r));
+        if (r.fp32 && !r.ho.at_floor && s_ != r.due && s_ % r.o.mixed_probe_every == 0) {
+            if (r.ho.observe(r.o, r.p.tolerance, s_, r.obs[which], r.due)) RC_TRY(promote(s, r));
         }
         if (s_ == r.due) {
             const double last = r.res;
             r.res = r.obs[which];
             r.F = r.obs[2];
-            if (r.fp32 && r.res <= r.o.mixed_switch) RC_TRY(promote(s, r));
             if (std::fabs(r.res - last) / last < 10E-16) {
                 RC_TRY(finish(s, r, GFC_ENOTCONVERGED));
                 continue;
@@ -449,8 +455,11 @@ int poll_once(gfc_server *s) {
                 continue;
             } else {
                 r.due = gfc_next_check((unsigned)s_, r.res, &r.p);
+                if (r.fp32 && r.ho.observe(r.o, r.p.tolerance, s_, r.res, r.due))
+                    RC_TRY(promote(s, r));
             }
         }
+        if (r.fp32 && s_ >= r.ho.switch_at) RC_TRY(promote(s, r));
         if ((unsigned)s_ >= r.p.max_sweeps_to_fix) {
             RC_TRY(finish(s, r, GFC_ENOTCONVERGED));
             continue;

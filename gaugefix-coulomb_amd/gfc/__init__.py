@@ -66,9 +66,11 @@ class _Options(C.Structure):
         ("precision", C.c_int),
         ("mixed_switch", C.c_double),
         ("mixed_probe_every", C.c_int),
+        ("mixed_defer", C.c_int),
         ("verbose", C.c_int),
         ("time_sweeps", C.c_int),
         ("retire_slices", C.c_int),
+        ("slice_group", C.c_int),
     ]
 
 
@@ -256,13 +258,16 @@ class Options:
     precision: int = PREC_FP64
     mixed_switch: float = 1e-9
     mixed_probe_every: int = 50
+    mixed_defer: int = 1
     verbose: int = 0
     time_sweeps: int = 0
     retire_slices: int = 0
+    slice_group: int = 0
 
     def _c(self) -> _Options:
-        return _Options(self.precision, self.mixed_switch, self.mixed_probe_every, self.verbose,
-                        self.time_sweeps, self.retire_slices)
+        return _Options(self.precision, self.mixed_switch, self.mixed_probe_every,
+                        self.mixed_defer, self.verbose, self.time_sweeps, self.retire_slices,
+                        self.slice_group)
 
 
 @dataclass

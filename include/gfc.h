@@ -123,6 +123,13 @@ typedef struct {
     double mixed_switch;    /* MIXED: FP32 -> FP64 hand-off residue (1e-9)     */
     int mixed_probe_every;  /* MIXED: half-sweeps between FP32-phase residue
                                probes (default 50); scheduled checks probe too */
+    int mixed_defer;        /* MIXED: 1 (default): once the FP32 phase has
+                               reached mixed_switch or its rounding floor, keep
+                               sweeping in FP32 until the sweep that leaves,
+                               before the next scheduled check, the FP64
+                               sweeps needed to reach the tolerance at the
+                               FP32 phase's convergence rate (x1.1 + 10);
+                               0: hand over at once                           */
     int verbose;            /* 1: print "Sweeps: ..." lines like the reference */
     int time_sweeps;        /* 1: bracket every sweep kernel with HIP events on
                                the job's stream and report their total in
@@ -135,6 +142,18 @@ typedef struct {
                                sites of one timeslice).  The global proxy is
                                still checked on the reference schedule.  Needs
                                ns a multiple of 8; not the drop-in default.    */
+    int slice_group;        /* timeslice blocking of the sweeps between two
+                               host decisions (checks, probes, max sweeps):
+                               the lattice is cut into groups of slice_group
+                               timeslices, and each group runs all those
+                               sweeps (with its reunitarizations) on one of
+                               four streams, concurrently with the other
+                               groups, so that kernel tails overlap.
+                               Timeslices do not interact, so the result is
+                               bit-identical to whole-lattice sweeps.
+                               0 (default): about 16 groups of at least 64K
+                               parity sites, or whole-lattice sweeps for
+                               small lattices; -1: whole-lattice sweeps.      */
 } gfc_options;
 
 typedef struct {

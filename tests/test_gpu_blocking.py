@@ -1,0 +1,58 @@
+"""Timeslice-blocked sweeps (gfc_options.slice_group).
+
+Between two host decisions the sweeps run group of timeslices by group of
+timeslices on several streams.  A site update reads only its own timeslice,
+so the gauge transformation, the sweep count and every residue must equal
+those of whole-lattice sweeps bit for bit, for any grouping (ragged last
+group, one group, one slice per group) and in both precisions.
+"""
+import numpy as np
+import pytest
+
+import gfc
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(job, params, options):
+    job.restore_config()
+    rc, rep = job.fix(params, options)
+    return rc, rep, job.download_gauge(), job.download_config()
+
+
+@pytest.mark.parametrize("precision", [gfc.PREC_FP64, gfc.PREC_MIXED])
+def test_blocked_equals_whole_lattice(ctx, precision):
+    geom = gfc.Geometry(8, 16)
+    p = gfc.Params(estimate_sweeps_to_gf_progress=400)
+    with gfc.Job(ctx, geom, precision) as job:
+        job.generate_smooth(5, 0.3)
+        job.reunitarize()
+        job.save_config()
+        o = dict(precision=precision, mixed_probe_every=30)
+        rc0, rep0, G0, U0 = _run(job, p, gfc.Options(slice_group=-1, **o))
+        assert rc0 > 0
+        for grp in (1, 3, 7, 16, 0):
+            rc, rep, G, U = _run(job, p, gfc.Options(slice_group=grp, **o))
+            print(f"precision {precision} group {grp}: {rc} sweeps, e2 {rep.residue_final:.3e}")
+            assert rc == rc0
+            assert rep.sweeps_fp32 == rep0.sweeps_fp32
+            assert rep.residue_final == rep0.residue_final
+            assert np.array_equal(G, G0)
+            assert np.array_equal(U, U0)
+
+
+def test_blocked_stop_at_max_sweeps(ctx):
+    # max sweeps (not a multiple of the reunitarization period) ends a block
+    geom = gfc.Geometry(8, 16)
+    p = gfc.Params(tolerance=1e-300, max_sweeps_to_fix=137, sweeps_to_reunitarization=10)
+    with gfc.Job(ctx, geom) as job:
+        job.generate_smooth(6, 0.3)
+        job.reunitarize()
+        job.save_config()
+        out = []
+        for grp in (-1, 5):
+            job.restore_config()
+            rc, rep = job.fix(p, gfc.Options(slice_group=grp))
+            assert rc == gfc.ENOTCONVERGED and rep.sweeps == 137
+            out.append(job.download_gauge())
+        assert np.array_equal(out[0], out[1])

@@ -231,3 +231,33 @@ def test_mixed_precision_reaches_tolerance(ctx):
     p1 = job.plaquette()
     assert abs(p1[0] - p0[0]) < PLAQ_ABS and abs(p1[1] - p0[1]) < PLAQ_ABS
     job.close()
+
+
+def test_mixed_deferred_handoff(ctx):
+    """Deferred hand-off (mixed_defer): on the reference check schedule the
+    FP32 phase runs past its switch residue up to the sweep that leaves enough
+    FP64 sweeps before the first check; the check passes at the same sweep as
+    in FP64, with more FP32 sweeps than the immediate hand-off, and the same
+    gauge (F to 1e-10 relative)."""
+    geom = gfc.Geometry(16, 32)
+    p = gfc.Params(sweeps_to_reunitarization=10, estimate_sweeps_to_gf_progress=1000)
+    with gfc.Job(ctx, geom, gfc.PREC_MIXED) as job:
+        job.generate_smooth(3, 0.35)
+        job.reunitarize()
+        job.save_config()
+        rc64, r64 = job.fix(p, gfc.Options(precision=gfc.PREC_FP64))
+        F64 = job.observables()[2]
+        res = {}
+        for defer in (0, 1):
+            job.restore_config()
+            rc, rep = job.fix(p, gfc.Options(precision=gfc.PREC_MIXED, mixed_defer=defer))
+            e2, th, F = job.observables()
+            print(f"defer {defer}: sweeps {rc} (fp32 {rep.sweeps_fp32}) e2 {e2:.3e} "
+                  f"F rel {abs(F - F64) / abs(F64):.2e}")
+            assert rc == rc64
+            assert e2 <= 1e-16
+            assert abs(F - F64) / abs(F64) < 1e-10
+            res[defer] = rep.sweeps_fp32
+        assert res[1] > res[0] > 0
+        # the FP64 tail still has to reach the tolerance before the check
+        assert rc64 - res[1] >= 100
