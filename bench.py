@@ -128,6 +128,66 @@ def profile_kernels(ctx, job, n: int):
         f"{2 * 2 ** 30 / sec / 1e9:.1f} GB/s read+write")
 
 
+def config5(ctx, ns: int, nt: int, eps: float, seed: int):
+    """64^3x128 (33.5 M sites): FP64, mixed, and mixed with per-timeslice
+    retirement on the reference schedule, plus the per-launch GB/s of the FP64
+    and FP32 sweep kernels (whole-lattice launches, HIP events).  The input is
+    synthesised in HBM by the device generator (the host generator's
+    construction), so no 19 GB host image is needed."""
+    import gfc
+    geom = gfc.Geometry(ns, nt)
+    vh = geom.volume // 2
+    params = gfc.Params(tolerance=1e-16, estimate_sweeps_to_gf_progress=1000,
+                        sweeps_to_reunitarization=10, omega_OR=1.95, hits=2)
+    out = {"lattice": f"{ns}^3x{nt}", "sites": geom.volume}
+    with gfc.Job(ctx, geom, gfc.PREC_MIXED) as job:
+        t = time.perf_counter()
+        job.generate_smooth(seed, eps)
+        job.reunitarize()
+        job.save_config()
+        log(f"config5: {ns}^3x{nt} synthesised in HBM in {time.perf_counter() - t:.2f}s")
+        plaq0 = job.plaquette()
+        runs = {"fp64": gfc.Options(precision=gfc.PREC_FP64),
+                "mixed": gfc.Options(precision=gfc.PREC_MIXED),
+                "mixed_retire": gfc.Options(precision=gfc.PREC_MIXED, retire_slices=1)}
+        F64 = None
+        for name, opt in runs.items():
+            job.restore_config()
+            t = time.perf_counter()
+            rc, rep = job.fix(params, opt)
+            dt = time.perf_counter() - t
+            e2, th, F = job.observables()
+            plaq = job.plaquette()
+            F64 = F if F64 is None else F64
+            out[name] = {"seconds": round(dt, 4), "sweeps": rc, "sweeps_fp32": rep.sweeps_fp32,
+                         "slice_sweeps": round(rep.slice_sweeps, 2), "final_e2": e2,
+                         "F_rel_diff_vs_fp64": abs(F - F64) / abs(F64),
+                         "plaquette_drift": max(abs(plaq[0] - plaq0[0]),
+                                                abs(plaq[1] - plaq0[1])),
+                         "site_updates_per_s": rep.slice_sweeps * vh / dt}
+            log(f"config5 {name}: {dt:.3f}s sweeps={rc} fp32={rep.sweeps_fp32} "
+                f"slice-sweeps={rep.slice_sweeps:.1f} e2={e2:.3e}")
+        out["mixed_speedup_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+        out["mixed_retire_speedup_vs_fp64"] = round(
+            out["fp64"]["seconds"] / out["mixed_retire"]["seconds"], 3)
+        fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
+                           max_sweeps_to_fix=40, sweeps_to_reunitarization=10 ** 6)
+        for name, opt, bpu in (("fp64", gfc.PREC_FP64, BYTES_PER_UPDATE_FP64),
+                               ("fp32", gfc.PREC_MIXED, BYTES_PER_UPDATE_FP32)):
+            job.restore_config()
+            rc, rep = job.fix(fixed, gfc.Options(precision=opt, mixed_switch=0.0,
+                                                 mixed_probe_every=10 ** 6, time_sweeps=1,
+                                                 slice_group=-1))
+            ms = rep.seconds_sweeps / rep.sweeps * 1e3
+            gbs = bpu * vh / (ms * 1e-3) / 1e9
+            out[f"{name}_sweep_kernel_ms"] = round(ms, 4)
+            out[f"{name}_sweep_GBs_algorithmic"] = round(gbs, 1)
+            out[f"{name}_sweep_frac_of_hbm_peak"] = round(gbs / HBM_PEAK_GBS, 4)
+            out[f"{name}_sweep_hbm_bytes_per_launch"] = pmc_traffic(ns, nt, name)
+            log(f"config5 {name} sweep kernel: {ms:.4f} ms/launch, {gbs:.1f} GB/s algorithmic")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,12 +200,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-config5", action="store_true",
+                    help="skip the 64^3x128 mixed/retirement leg")
     ap.add_argument("--serve", type=int, default=256, metavar="N",
                     help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--profile-kernels", type=int, default=0, metavar="N",
                     help="only run N FP64 and N FP32 half-sweeps plus a 1 GiB device copy "
                          "(calibration for rocprofv3 PMC counters), then exit")
+    ap.add_argument("--device-input", action="store_true",
+                    help="with --profile-kernels: synthesise the input in HBM")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this engine runs one lattice per GPU and the "
@@ -157,6 +221,15 @@ def main():
     geom = gfc.Geometry(a.ns, a.nt)
     V = geom.volume
     vh = V // 2
+    if a.profile_kernels and a.device_input:
+        ctx = gfc.Context(0)
+        with gfc.Job(ctx, geom, gfc.PREC_MIXED) as job:
+            job.generate_smooth(a.seed, a.eps)
+            job.reunitarize()
+            job.save_config()
+            profile_kernels(ctx, job, a.profile_kernels)
+        ctx.close()
+        return
     log(f"generating {a.ns}^3x{a.nt} eps={a.eps} seed={a.seed}")
     t = time.time()
     U = gfc.generate_smooth(geom, a.seed, a.eps)
@@ -275,6 +348,9 @@ def main():
                                  "48^3x96: 10%}, eps U[0.2,0.4], tol {1e-14, 1e-16}, "
                                  f"Poisson {a.serve_rate}/s, inputs synthesised in HBM")
         log(f"serve: {serve_out}")
+    c5 = None
+    if not a.no_config5:
+        c5 = config5(ctx, 64, 128, a.eps, a.seed)
     ctx.close()
 
     cpu = None
@@ -313,6 +389,7 @@ def main():
         "copy_kernel_GBs": round(copy_gbs, 1),
         "sweep_frac_of_copy": round(achieved / copy_gbs, 4),
         "serve": serve_out,
+        "config5": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)
