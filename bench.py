@@ -128,6 +128,42 @@ def profile_kernels(ctx, job, n: int):
         f"{2 * 2 ** 30 / sec / 1e9:.1f} GB/s read+write")
 
 
+def io_leg(job, geom, U):
+    """Load of the FP32 little-endian input file (hot page cache): the device
+    stream (gfc_job_load_config: parallel pread into pinned blocks, DMA,
+    conversion on the device) against the host loader (gfc_load_config) plus
+    the upload of the FP64 image."""
+    import tempfile
+    import gfc
+    d = tempfile.mkdtemp(dir=os.environ.get("TMPDIR", "/tmp"))
+    path = os.path.join(d, "cfg_fp32_le.bin")
+    try:
+        if gfc.write_config(path, geom, U, 4, False) != 0:
+            return None
+        nbytes = os.path.getsize(path)
+        t = time.perf_counter()
+        rc = job.load_config(path, 4, False)
+        t_dev = time.perf_counter() - t
+        assert rc == 0, rc
+        Ud = job.download_config()
+        t = time.perf_counter()
+        rc, Uh = gfc.load_config(path, geom, 4, False)
+        job.upload_config(Uh)
+        t_host = time.perf_counter() - t
+        same = bool(np.array_equal(Ud, Uh))
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+        os.rmdir(d)
+    out = {"file": "FP32 little-endian, hot page cache", "bytes": nbytes,
+           "device_stream_seconds": round(t_dev, 4),
+           "device_stream_GBs": round(nbytes / t_dev / 1e9, 2),
+           "host_load_plus_upload_seconds": round(t_host, 4),
+           "same_result": same}
+    log(f"io: {out}")
+    return out
+
+
 def config5(ctx, ns: int, nt: int, eps: float, seed: int):
     """64^3x128 (33.5 M sites): FP64, mixed, and mixed with per-timeslice
     retirement on the reference schedule, plus the per-launch GB/s of the FP64
@@ -200,6 +236,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-io", action="store_true", help="skip the file-load leg")
     ap.add_argument("--no-config5", action="store_true",
                     help="skip the 64^3x128 mixed/retirement leg")
     ap.add_argument("--serve", type=int, default=256, metavar="N",
@@ -333,6 +370,9 @@ def main():
             f"(fp32 {rep_i.sweeps_fp32}) speedup {step_s / imm_s:.3f}; fp32 sweep {sweep32_ms:.4f} ms; dense schedule: "
             f"fp64 {d64_s:.3f}s ({d64.sweeps}) mixed {dmx_s:.3f}s ({dmx.sweeps}, "
             f"fp32 {dmx.sweeps_fp32}) speedup {d64_s / dmx_s:.3f}")
+    io = None
+    if not a.no_io:
+        io = io_leg(job, geom, U)
     job.close()
 
     serve_out = None
@@ -390,6 +430,7 @@ def main():
         "sweep_frac_of_copy": round(achieved / copy_gbs, 4),
         "serve": serve_out,
         "config5": c5,
+        "io": io,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

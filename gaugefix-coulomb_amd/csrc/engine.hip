@@ -85,6 +85,10 @@ void gfc_ctx_destroy(gfc_ctx *c) {
             hipStreamSynchronize(c->lane[k]);
             hipStreamDestroy(c->lane[k]);
         }
+    for (int k = 0; k < 2; k++) {
+        if (c->pin[k]) hipHostFree(c->pin[k]);
+        if (c->pin_ev[k]) hipEventDestroy(c->pin_ev[k]);
+    }
     if (c->fork) hipEventDestroy(c->fork);
     for (int k = 0; k < gfc_ctx::kLanes; k++)
         if (c->join[k]) hipEventDestroy(c->join[k]);

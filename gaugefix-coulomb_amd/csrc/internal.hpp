@@ -58,6 +58,9 @@ struct gfc_ctx {
     hipStream_t lane[kLanes] = {};
     hipEvent_t fork = nullptr;
     hipEvent_t join[kLanes] = {};
+    // pinned host blocks of the file streams (io_device.hip), made on first use
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
 
 struct gfc_job {

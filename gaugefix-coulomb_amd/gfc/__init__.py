@@ -163,6 +163,9 @@ def _load():
         "gfc_reunitarize_field": (I, [P, D, I64]),
         "gfc_config_observables": (I, [P, C.POINTER(_Geom), D, D, D, D, D, D]),
         "gfc_job_generate_smooth": (I, [P, C.c_uint64, C.c_double]),
+        "gfc_job_load_config": (I, [P, C.c_char_p, I, I]),
+        "gfc_job_write_gauge_transf": (I, [P, C.c_char_p]),
+        "gfc_job_write_config": (I, [P, C.c_char_p, I, I]),
         "gfc_server_create": (I, [P, I64, I, I, C.POINTER(P)]),
         "gfc_server_destroy": (None, [P]),
         "gfc_server_clock": (C.c_double, [P]),
@@ -452,6 +455,26 @@ class Job:
     def generate_smooth(self, seed: int, eps: float):
         """Synthesise the smooth configuration directly in HBM (G <- 1)."""
         _check(lib.gfc_job_generate_smooth(self._p, seed, eps))
+
+    def load_config(self, path: str, in_precision: int = 8, byte_swap: bool = True) -> int:
+        """Stream a configuration file into the job (loadConfig rules); returns
+        the loader's code: 0, -1 open, -2 short, -3 trailing bytes, -4 precision."""
+        rc = lib.gfc_job_load_config(self._p, os.fsencode(path), in_precision, int(byte_swap))
+        if rc < -4:
+            _check(rc)
+        return rc
+
+    def write_gauge_transf(self, path: str) -> int:
+        rc = lib.gfc_job_write_gauge_transf(self._p, os.fsencode(path))
+        if rc < -4:
+            _check(rc)
+        return rc
+
+    def write_config(self, path: str, out_precision: int = 8, byte_swap: bool = True) -> int:
+        rc = lib.gfc_job_write_config(self._p, os.fsencode(path), out_precision, int(byte_swap))
+        if rc < -4:
+            _check(rc)
+        return rc
 
     def save_config(self):
         _check(lib.gfc_job_save_config(self._p))

@@ -233,6 +233,22 @@ int gfc_job_restore_config(gfc_job *job);
  * (what gauge_fix_coulomb.c:120,132 prints with printMatrix3x3). */
 int gfc_job_get_link(gfc_job *job, int64_t site, int mu, double *out18);
 
+/* ---- file streams between disk and the device ---------------------------- */
+/* loadConfig (src/fields_io.c:216-305; FP32 input as convertCfg_in_work,
+ * src/convert_precision_io.c:176-215) straight into the job: the file is read
+ * in pinned blocks by parallel pread while the previous block is copied to the
+ * device, then byte-swapped (byte_swap != 0) and converted on the device.  G
+ * is reset to the identity.  Same file rules and return codes as
+ * gfc_load_config: 0, -1 (open), -2 (short file or read error), -3 (trailing
+ * bytes), -4 (bad precision); regular files only. */
+int gfc_job_load_config(gfc_job *job, const char *path, int in_precision, int byte_swap);
+/* writeGaugeTransf (src/fields_io.c:493-565) of the job's G: native-endian
+ * FP64, V*144 bytes.  0, -1 (create), -2 (write error). */
+int gfc_job_write_gauge_transf(gfc_job *job, const char *path);
+/* writeConfig (src/fields_io.c:405-489) of the job's current U, FP64 or FP32
+ * (out_precision 8 or 4), optionally byte-swapped.  0, -1, -2, -4. */
+int gfc_job_write_config(gfc_job *job, const char *path, int out_precision, int byte_swap);
+
 /* The drop-in entry point: gaugefixOverrelaxation(U, G, params) on host
  * arrays (include/gauge_fixing.h:67-69).  U (4*V*18 doubles) and G (V*18
  * doubles) are updated in place with the same return codes.  o and rep may
