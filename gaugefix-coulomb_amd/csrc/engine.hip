@@ -388,16 +388,16 @@ int half_sweep(gfc_job *j, SweepState &st, int parity, double omega, int hits) {
     return GFC_OK;
 }
 
-// Projects G on parity indices [h0, h0 + count) of both parities of the live
-// array.
-int reunit_range(gfc_job *j, hipStream_t s, bool fp32, int h0, int count) {
-    const unsigned b = nblocks(2 * (int64_t)count);
+// Projects G on parity indices [h0, h0 + count) of parities [p0, p0 + np)
+// of the live array.
+int reunit_range(gfc_job *j, hipStream_t s, bool fp32, int p0, int np, int h0, int count) {
+    const unsigned b = nblocks((int64_t)np * count);
     if (fp32)
-        hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh, 0,
-                           2, h0, count);
+        hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh, p0,
+                           np, h0, count);
     else
-        hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh, 0,
-                           2, h0, count);
+        hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh, p0,
+                           np, h0, count);
     LAUNCH_CHECK();
     return GFC_OK;
 }
@@ -430,9 +430,23 @@ int group_size(const gfc_job *j, const gfc_options &o) {
     return 2 * n <= nt ? (int)n : 0;
 }
 
+template <typename T>
+int launch_sweep(const cplx<T> *u0, cplx<T> *g, hipStream_t s, int ns, int Vh, int p, T omega,
+                 int hits, int h0, int count, bool pout) {
+    const unsigned b = nblocks(count);
+    if (pout)
+        hipLaunchKernelGGL((gfc::k_sweep<T, true>), dim3(b), dim3(kBlock), 0, s, u0, g, ns, Vh,
+                           p, omega, hits, h0, count);
+    else
+        hipLaunchKernelGGL((gfc::k_sweep<T>), dim3(b), dim3(kBlock), 0, s, u0, g, ns, Vh, p,
+                           omega, hits, h0, count);
+    LAUNCH_CHECK();
+    return GFC_OK;
+}
+
 // Half-sweeps [from, to) (parity = sweep number & 1), timeslice group by
 // timeslice group, with the reunitarization of G after every sweep number
-// s + 1 < to that is a multiple of R.  Every site update reads only sites of
+// s + 1 < to that is a multiple of R (folded into the sweeps around it).  Every site update reads only sites of
 // its own timeslice, so each group runs through the sweeps on its own and the
 // result equals that of whole-lattice sweeps bit for bit.
 int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group, double omega,
@@ -451,21 +465,21 @@ int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group,
     for (int g0 = 0; g0 < ngroups; g0 += lanes) {
         const int g1 = std::min(ngroups, g0 + lanes);
         for (int k = from; k < to; k++) {
+            // a reunitarization after sweep k + 1 < to: sweep k projects its
+            // own parity on output, a pass over the group the other parity
+            const bool pout = k + 1 < to && (unsigned)(k + 1) % R == 0;
             for (int gi = g0; gi < g1; gi++) {
                 hipStream_t s = c->lane[gi - g0];
                 const int t0 = gi * group;
                 const int h0 = t0 * S;
                 const int count = (t0 + group <= nt ? group : nt - t0) * S;
-                const unsigned b = nblocks(count);
                 if (st.fp32)
-                    hipLaunchKernelGGL(gfc::k_sweep<float>, dim3(b), dim3(kBlock), 0, s, j->u0f,
-                                       j->gf, ns, Vh, k & 1, (float)omega, hits, h0, count);
+                    RC_TRY(launch_sweep(j->u0f, j->gf, s, ns, Vh, k & 1, (float)omega, hits, h0,
+                                        count, pout));
                 else
-                    hipLaunchKernelGGL(gfc::k_sweep<double>, dim3(b), dim3(kBlock), 0, s, j->u0,
-                                       j->g, ns, Vh, k & 1, omega, hits, h0, count);
-                LAUNCH_CHECK();
-                if (k + 1 < to && (unsigned)(k + 1) % R == 0)
-                    RC_TRY(reunit_range(j, s, st.fp32, h0, count));
+                    RC_TRY(launch_sweep(j->u0, j->g, s, ns, Vh, k & 1, omega, hits, h0, count,
+                                        pout));
+                if (pout) RC_TRY(reunit_range(j, s, st.fp32, 1 - (k & 1), 1, h0, count));
             }
         }
     }

@@ -177,7 +177,10 @@ __device__ __forceinline__ Nbr neighbours(const Site &s, int h, int ns) {
 // The site update of parity index h on parity p: reads 6 u0 links, 6
 // neighbour G and G(n); writes G(n).  Shared by the single-lattice and the
 // batched (serving) kernels, so a site computes the same bits in both.
-template <typename T>
+// POUT projects the new G(n) onto SU(3) before storing it: the same bits as a
+// reunitarization of this parity's G right after the sweep, since the
+// projection reads rows 0 and 1 only.
+template <typename T, bool POUT = false>
 __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
                                            cplx<T> *__restrict__ g, int ns, int Vh, int p,
                                            T omega, int hits, int h) {
@@ -202,18 +205,19 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
     m_mul(gn, W, w);
     la_or_update(w, hits, omega, upd);
     m_mul(upd, gn, out);
+    if (POUT) m_project_su3(out);
     store_su3<true>(out, gp, h);
 }
 
 // One red/black half-sweep on parity p over parity indices [h0, h0 + count).
-template <typename T>
+template <typename T, bool POUT = false>
 __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0,
                                                   cplx<T> *__restrict__ g, int ns, int Vh,
                                                   int p, T omega, int hits, int h0,
                                                   int count) {
     const int tid = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
     if (tid >= count) return;
-    sweep_site<T>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
+    sweep_site<T, POUT>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
 }
 
 // ---------------------------------------------------------------------------
