@@ -462,34 +462,40 @@ int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group,
     }
     // `lanes` groups at a time, one per stream, launched in step so that the
     // streams stay busy together
-    for (int g0 = 0; g0 < ngroups; g0 += lanes) {
-        const int g1 = std::min(ngroups, g0 + lanes);
-        for (int k = from; k < to; k++) {
-            // a reunitarization after sweep k + 1 < to: sweep k projects its
-            // own parity on output, a pass over the group the other parity
-            const bool pout = k + 1 < to && (unsigned)(k + 1) % R == 0;
-            for (int gi = g0; gi < g1; gi++) {
-                hipStream_t s = c->lane[gi - g0];
-                const int t0 = gi * group;
-                const int h0 = t0 * S;
-                const int count = (t0 + group <= nt ? group : nt - t0) * S;
-                if (st.fp32)
-                    RC_TRY(launch_sweep(j->u0f, j->gf, s, ns, Vh, k & 1, (float)omega, hits, h0,
-                                        count, pout));
-                else
-                    RC_TRY(launch_sweep(j->u0, j->g, s, ns, Vh, k & 1, omega, hits, h0, count,
-                                        pout));
-                if (pout) RC_TRY(reunit_range(j, s, st.fp32, 1 - (k & 1), 1, h0, count));
+    auto launch_all = [&]() -> int {
+        for (int g0 = 0; g0 < ngroups; g0 += lanes) {
+            const int g1 = std::min(ngroups, g0 + lanes);
+            for (int k = from; k < to; k++) {
+                // a reunitarization after sweep k + 1 < to: sweep k projects its
+                // own parity on output, a pass over the group the other parity
+                const bool pout = k + 1 < to && (unsigned)(k + 1) % R == 0;
+                for (int gi = g0; gi < g1; gi++) {
+                    hipStream_t s = c->lane[gi - g0];
+                    const int t0 = gi * group;
+                    const int h0 = t0 * S;
+                    const int count = (t0 + group <= nt ? group : nt - t0) * S;
+                    if (st.fp32)
+                        RC_TRY(launch_sweep(j->u0f, j->gf, s, ns, Vh, k & 1, (float)omega, hits,
+                                            h0, count, pout));
+                    else
+                        RC_TRY(launch_sweep(j->u0, j->g, s, ns, Vh, k & 1, omega, hits, h0,
+                                            count, pout));
+                    if (pout) RC_TRY(reunit_range(j, s, st.fp32, 1 - (k & 1), 1, h0, count));
+                }
             }
         }
-    }
+        return GFC_OK;
+    };
+    const int rc = launch_all();
+    // the main stream waits for the other streams also after a failed launch,
+    // so that no work of this call is left behind it
     if (lanes > 1) {
         for (int l = 1; l < lanes; l++) {
             HIP_TRY(hipEventRecord(c->join[l], c->lane[l]));
             HIP_TRY(hipStreamWaitEvent(c->stream, c->join[l], 0));
         }
     }
-    return GFC_OK;
+    return rc;
 }
 
 int promote_g(gfc_job *j, SweepState &st) {
