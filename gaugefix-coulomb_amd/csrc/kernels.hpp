@@ -31,15 +31,11 @@ constexpr int kPlanes = 6;  // stored complex elements per SU(3) matrix
 // that launches them (engine.hip, server.hip) carries its own code object.
 namespace {
 
-// Workgroups are dispatched round-robin over the 8 XCDs (one L2 each); remap
-// so that each XCD walks one contiguous range of sites and the neighbour
-// reads of adjacent z-planes hit the same L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
-    const unsigned nx = 8;
-    const unsigned q = nblocks / nx, r = nblocks % nx;
-    const unsigned x = bid % nx, i = bid / nx;
-    return x * q + (x < r ? x : r) + i;
-}
+// Workgroups are dispatched round-robin over the 8 XCDs and keep that order:
+// the workgroups running at any moment cover one contiguous stretch of sites,
+// which the sweep kernels stream through together.  (Remapping each XCD onto
+// its own contiguous range, so that z-neighbour planes share one L2, measured
+// 5% slower at 48^3 x 96.)
 
 // Loads that stream (read once per sweep) carry the non-temporal hint so that
 // the L2 keeps the neighbour G matrices, which six sites read.
@@ -215,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0
                                                   cplx<T> *__restrict__ g, int ns, int Vh,
                                                   int p, T omega, int hits, int h0,
                                                   int count) {
-    const int tid = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+    const int tid = (int)(blockIdx.x * kBlock + threadIdx.x);
     if (tid >= count) return;
     sweep_site<T, POUT>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
 }
@@ -649,7 +645,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sweep_batch(const BatchJob *__restrict__ jobs,
                                                         const int4 *__restrict__ items,
                                                         int nitems, int per_block) {
-    const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int b = (int)blockIdx.x;
     const int i1 = min((b + 1) * per_block, nitems);
     for (int i = b * per_block; i < i1; i++) {
         const int4 it = items[i];
