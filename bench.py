@@ -53,6 +53,25 @@ def pmc_traffic(ns: int, nt: int, tag: str):
     return float(m.group(1)) if m else None
 
 
+class _CStdoutToStderr:
+    """Sends file descriptor 1 to stderr while the CPU baseline runs: the
+    reference library prints its "Sweeps:" log with printf, and stdout must
+    carry only the JSON record."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.libc = C.CDLL(None)
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        self.libc.fflush(None)
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
 def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
     """Times the reference's gaugefixOverrelaxation (oracle/_ref build) on n
     half-sweeps of the same lattice; falls back to the C oracle port."""
@@ -99,10 +118,11 @@ def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
             orc.gaugefix(Uw, G, g, p)
             return time.perf_counter() - t
         kind = "port"
-    t1 = run(1)  # setup (G <- 1, initial e2) + one half-sweep
-    per = max(t1 * 0.5, 1e-3)
-    n = max(2, min(40, int(seconds / per)))
-    tn = run(1 + n)
+    with _CStdoutToStderr():
+        t1 = run(1)  # setup (G <- 1, initial e2) + one half-sweep
+        per = max(t1 * 0.5, 1e-3)
+        n = max(2, min(40, int(seconds / per)))
+        tn = run(1 + n)
     per_sweep = max(tn - t1, 1e-9) / n
     return {"value": vh / per_sweep, "unit": "site-updates/s", "cores": threads,
             "kind": kind,
