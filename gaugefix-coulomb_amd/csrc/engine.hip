@@ -172,8 +172,9 @@ int ensure_partials(gfc_ctx *c, size_t blocks) {
     return GFC_OK;
 }
 
-int set_identity(gfc_job *j) {
-    hipStream_t s = j->ctx->stream;
+int set_identity(gfc_job *j) { return set_identity(j, j->ctx->stream); }
+
+int set_identity(gfc_job *j, hipStream_t s) {
     const unsigned b = nblocks(2 * (int64_t)j->Vh);
     hipLaunchKernelGGL(gfc::k_g_identity<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh);
     LAUNCH_CHECK();
@@ -252,7 +253,10 @@ extern "C" int gfc_job_upload_config(gfc_job *j, const void *U, int in_precision
 }
 
 int gfcx::generate_smooth_device(gfc_job *j, uint64_t seed, double eps) {
-    hipStream_t s = j->ctx->stream;
+    return generate_smooth_device(j, j->ctx->stream, seed, eps);
+}
+
+int gfcx::generate_smooth_device(gfc_job *j, hipStream_t s, uint64_t seed, double eps) {
     // base state of the draws, as in host_gen.c: mix64(seed + golden gamma)
     uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -265,7 +269,7 @@ int gfcx::generate_smooth_device(gfc_job *j, uint64_t seed, double eps) {
     hipLaunchKernelGGL(gfc::k_gen_links, dim3(nblocks(j->V)), dim3(kBlock), 0, s, base, eps,
                        j->geom.ns, j->geom.nt, tmp, j->cfg);
     LAUNCH_CHECK();
-    return set_identity(j);
+    return set_identity(j, s);
 }
 
 extern "C" int gfc_job_generate_smooth(gfc_job *j, uint64_t seed, double eps) {
@@ -347,8 +351,9 @@ int obs_aos(gfc_job *j, double out[3]) {
     return GFC_OK;
 }
 
-int build_u0(gfc_job *j) {
-    hipStream_t s = j->ctx->stream;
+int build_u0(gfc_job *j) { return build_u0(j, j->ctx->stream); }
+
+int build_u0(gfc_job *j, hipStream_t s) {
     const unsigned b = nblocks(2 * (int64_t)j->Vh);
     hipLaunchKernelGGL(gfc::k_build_u0<double>, dim3(b), dim3(kBlock), 0, s, j->cfg, j->u0,
                        j->geom.ns, j->Vh);
@@ -361,8 +366,9 @@ int build_u0(gfc_job *j) {
     return GFC_OK;
 }
 
-int materialize(gfc_job *j) {
-    hipStream_t s = j->ctx->stream;
+int materialize(gfc_job *j) { return materialize(j, j->ctx->stream); }
+
+int materialize(gfc_job *j, hipStream_t s) {
     hipLaunchKernelGGL(gfc::k_materialize, dim3(nblocks(j->V)), dim3(kBlock), 0, s, j->cfg, j->g,
                        j->geom.ns, j->geom.nt, j->Vh);
     LAUNCH_CHECK();
@@ -404,7 +410,10 @@ int reunit_range(gfc_job *j, hipStream_t s, bool fp32, int p0, int np, int h0, i
 
 // Projects G on parities [p0, p0 + np) of the live array.
 int reunit_g(gfc_job *j, bool fp32, int p0, int np) {
-    hipStream_t s = j->ctx->stream;
+    return reunit_g(j, j->ctx->stream, fp32, p0, np);
+}
+
+int reunit_g(gfc_job *j, hipStream_t s, bool fp32, int p0, int np) {
     const unsigned b = nblocks((int64_t)np * j->Vh);
     if (fp32)
         hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh, p0,
@@ -498,20 +507,20 @@ int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group,
     return rc;
 }
 
-int promote_g(gfc_job *j, SweepState &st) {
-    hipStream_t s = j->ctx->stream;
+int promote_g_only(gfc_job *j, hipStream_t s) {
     const unsigned b = nblocks(2 * (int64_t)j->Vh);
     hipLaunchKernelGGL(gfc::k_g_promote, dim3(b), dim3(kBlock), 0, s, j->gf, j->g, j->Vh);
     LAUNCH_CHECK();
+    return GFC_OK;
+}
+
+int promote_g(gfc_job *j, SweepState &st) {
+    RC_TRY(promote_g_only(j, j->ctx->stream));
     st.fp32 = false;
     return GFC_OK;
 }
 
-int promote_g_only(gfc_job *j) {
-    SweepState st;
-    st.fp32 = true;
-    return promote_g(j, st);
-}
+int promote_g_only(gfc_job *j) { return promote_g_only(j, j->ctx->stream); }
 
 struct SweepTimer {
     bool on = false;

@@ -126,13 +126,20 @@ struct Handoff {
     }
 };
 
-// Helpers of engine.hip used by the scheduler.  All launch on j->ctx->stream.
+// Helpers of engine.hip used by the scheduler.  They launch on the given
+// stream, or on j->ctx->stream without one.
 int set_identity(gfc_job *j);
+int set_identity(gfc_job *j, hipStream_t s);
 int build_u0(gfc_job *j);
+int build_u0(gfc_job *j, hipStream_t s);
 int materialize(gfc_job *j);
+int materialize(gfc_job *j, hipStream_t s);
 int reunit_g(gfc_job *j, bool fp32, int p0, int np);
+int reunit_g(gfc_job *j, hipStream_t s, bool fp32, int p0, int np);
 int promote_g_only(gfc_job *j);
+int promote_g_only(gfc_job *j, hipStream_t s);
 int generate_smooth_device(gfc_job *j, uint64_t seed, double eps);
+int generate_smooth_device(gfc_job *j, hipStream_t s, uint64_t seed, double eps);
 int ensure_partials(gfc_ctx *c, size_t blocks);
 
 }  // namespace gfcx

@@ -4,18 +4,26 @@
 // A request is admitted when a job slot and its HBM footprint are free: its
 // configuration is synthesised in HBM, reunitarized, and its fixing state is
 // the one gfc_job_fix keeps (sweep count, check schedule, FP32/FP64 phase).
-// Every poll advances all resident lattices together:
-//   1. admit what fits;
+//
+// Lattices are served in four lanes by size (fewer than 2^18, 2^20, 2^22
+// sites, and larger), one stream each, so that a small lattice is never
+// stepped in lockstep with a large one: its half-sweeps take microseconds
+// while a large lattice's take a millisecond, and the streams share the GPU.
+// A lane's poll:
+//   1. admits the queued requests of its size class that fit;
 //   2. one batched residue evaluation for every lattice at a check, a mixed-
-//      precision probe or its start (one host synchronisation);
+//      precision probe or its start (one synchronisation of the lane);
 //   3. the decisions of gaugefixOverrelaxation (src/gauge_fixing.c:771-820)
 //      per lattice: stall, convergence, next check, max sweeps,
 //      reunitarization; finished lattices are materialised and retired;
 //   4. a chunk of half-sweeps: each lattice runs up to its next event (next
-//      multiple of sweeps_to_reunitarization, next check, next probe, max),
-//      and every half-sweep step is ONE launch per precision that packs the
-//      256-site chunks of all lattices (k_sweep_batch).  The chunk's launch
-//      sequence is captured in a HIP graph and replayed while the plan repeats.
+//      multiple of sweeps_to_reunitarization, next check, next probe, hand-off,
+//      max), and every half-sweep step is ONE launch per precision that packs
+//      the 256-site chunks of the lane's lattices (k_sweep_batch).  The chunk's
+//      launch sequence is captured in a HIP graph and replayed while the plan
+//      repeats.  At most two chunks of a lane are in flight.
+// gfc_server_poll serves, in turn, every lane that can proceed without
+// waiting for the GPU, so a busy lane never holds up the others.
 // Per-site arithmetic and reduction order are those of the single-lattice
 // path, so a lattice gets the same result alone or in the mix.
 #include <hip/hip_runtime.h>
@@ -27,6 +35,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -41,6 +50,14 @@ namespace {
 
 enum State { QUEUED, INIT, RUN, DONE };
 
+constexpr int kLanes = gfc_ctx::kLanes;
+
+// Size class of a lattice: its lane.
+int lane_of(const gfc_geom &g) {
+    const int64_t V = gfc_geom_volume(&g);
+    return V < (1 << 18) ? 0 : V < (1 << 20) ? 1 : V < (1 << 22) ? 2 : 3;
+}
+
 struct Req {
     int64_t id = 0;
     uint64_t tag = 0;
@@ -53,11 +70,12 @@ struct Req {
     gfc_job *job = nullptr;
     int64_t footprint = 0;
     int prec = GFC_PREC_FP64;
+    int lane = 0;
     State state = QUEUED;
     // fixing state, as in run_fix (engine.hip)
     int sweep = 0, due = 0, evals = 0, sweeps_fp32 = 0, rc = 0;
     double res = 0, res0 = 0, F = 0;
-    bool fp32 = false, finished_early = false;
+    bool fp32 = false;
     gfcx::Handoff ho;
     int slot = -1;
     double obs[3] = {0, 0, 0};
@@ -70,6 +88,29 @@ T *dev_alloc(size_t n) {
     return p;
 }
 
+// One size class: its stream, resident lattices and device-side tables.
+struct Lane {
+    hipStream_t st = nullptr;
+    std::vector<std::unique_ptr<Req>> active;
+    BatchJob *d_jobs = nullptr;
+    BatchJob *h_jobs = nullptr;  // pinned, max_jobs descriptors by slot
+    // sweep items and graphs of the chunk plans, double-buffered: a new plan
+    // is written into the buffer the chunk in flight does not use
+    int4 *d_items[2] = {nullptr, nullptr}, *h_items[2] = {nullptr, nullptr};
+    size_t items_cap[2] = {0, 0};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int cur = 0;  // buffer of the current plan
+    int4 *d_oitems = nullptr, *h_oitems = nullptr;  // residue-evaluation items
+    size_t oitems_cap = 0;
+    int *d_sel = nullptr, *h_sel = nullptr;
+    double *d_part = nullptr;
+    size_t part_cap = 0;
+    double *d_out = nullptr, *h_out = nullptr;
+    std::vector<int64_t> plan_sig;
+    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+    int64_t chunks = 0;
+};
+
 }  // namespace
 
 struct gfc_server {
@@ -79,28 +120,19 @@ struct gfc_server {
     std::chrono::steady_clock::time_point t0;
     int64_t next_id = 1;
     std::deque<std::unique_ptr<Req>> queue;
-    std::vector<std::unique_ptr<Req>> active;
+    Lane lanes[kLanes];
     std::map<int64_t, std::unique_ptr<Req>> kept;
     std::multimap<std::tuple<int, int, int>, gfc_job *> cache;  // (ns, nt, prec) -> free job
     std::vector<gfc_completion> done;
-    // device-side scheduling tables
-    BatchJob *d_jobs = nullptr;
-    BatchJob *h_jobs = nullptr;  // pinned
-    int4 *d_items = nullptr, *h_items = nullptr;  // sweep items of the current chunk plan
-    size_t items_cap = 0;
-    int4 *d_oitems = nullptr, *h_oitems = nullptr;  // residue-evaluation items
-    size_t oitems_cap = 0;
-    int *d_sel = nullptr, *h_sel = nullptr;
-    double *d_part = nullptr;
-    size_t part_cap = 0;
-    double *d_out = nullptr, *h_out = nullptr;
-    // chunk graph
-    hipGraphExec_t graph = nullptr;
-    std::vector<int64_t> plan_sig;
-    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+    hipEvent_t created = nullptr;  // job creation on ctx->stream, awaited by the lanes
     gfc_server_stats stats{};
     double clock() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    int n_active() const {
+        int n = 0;
+        for (const Lane &L : lanes) n += (int)L.active.size();
+        return n;
     }
 };
 
@@ -108,10 +140,10 @@ namespace {
 
 int64_t footprint_of(const Req &r) { return gfc_job_footprint(&r.geom, r.prec); }
 
-int grow_table(gfc_server *s, int4 *&d, int4 *&h, size_t &cap, size_t n) {
+int grow_table(Lane &L, int4 *&d, int4 *&h, size_t &cap, size_t n) {
     if (n <= cap) return GFC_OK;
     const size_t c = std::max(n, 2 * cap);
-    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(L.st));
     hipFree(d);
     hipHostFree(h);
     d = nullptr;
@@ -123,18 +155,19 @@ int grow_table(gfc_server *s, int4 *&d, int4 *&h, size_t &cap, size_t n) {
     return GFC_OK;
 }
 
-int grow_part(gfc_server *s, size_t blocks) {
-    if (blocks <= s->part_cap) return GFC_OK;
-    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
-    hipFree(s->d_part);
-    s->d_part = nullptr;
-    HIP_TRY(hipMalloc(&s->d_part, blocks * 3 * sizeof(double)));
-    s->part_cap = blocks;
+int grow_part(Lane &L, size_t blocks) {
+    if (blocks <= L.part_cap) return GFC_OK;
+    HIP_TRY(hipStreamSynchronize(L.st));
+    hipFree(L.d_part);
+    L.d_part = nullptr;
+    L.part_cap = 0;
+    HIP_TRY(hipMalloc(&L.d_part, blocks * 3 * sizeof(double)));
+    L.part_cap = blocks;
     return GFC_OK;
 }
 
-void fill_desc(gfc_server *s, Req &r) {
-    BatchJob &b = s->h_jobs[r.slot];
+void fill_desc(Lane &L, Req &r) {
+    BatchJob &b = L.h_jobs[r.slot];
     gfc_job *j = r.job;
     b.u0 = r.fp32 ? (const void *)j->u0f : (const void *)j->u0;
     b.g = r.fp32 ? (void *)j->gf : (void *)j->g;
@@ -147,16 +180,17 @@ void fill_desc(gfc_server *s, Req &r) {
     b.hits = (int)r.p.hits;
 }
 
-int upload_desc(gfc_server *s) {
-    HIP_TRY(hipMemcpyAsync(s->d_jobs, s->h_jobs, sizeof(BatchJob) * s->max_jobs,
-                           hipMemcpyHostToDevice, s->ctx->stream));
+int upload_desc(gfc_server *s, Lane &L) {
+    HIP_TRY(hipMemcpyAsync(L.d_jobs, L.h_jobs, sizeof(BatchJob) * s->max_jobs,
+                           hipMemcpyHostToDevice, L.st));
     return GFC_OK;
 }
 
-// Takes a free slot and a job (from the cache or newly allocated) for r.
-int admit_one(gfc_server *s, Req &r, bool &admitted) {
+// Takes a free slot of lane L and a job (from the cache or newly allocated)
+// for r.
+int admit_one(gfc_server *s, Lane &L, Req &r, bool &admitted) {
     admitted = false;
-    if ((int)s->active.size() >= s->max_jobs) return GFC_OK;
+    if (s->n_active() >= s->max_jobs) return GFC_OK;
     auto key = std::make_tuple(r.geom.ns, r.geom.nt, r.prec);
     auto it = s->cache.find(key);
     gfc_job *job = nullptr;
@@ -174,88 +208,91 @@ int admit_one(gfc_server *s, Req &r, bool &admitted) {
         if (s->used + r.footprint > s->budget) return GFC_OK;  // wait for memory
         RC_TRY(gfc_job_create(s->ctx, &r.geom, r.prec, &job));
         s->used += r.footprint;
+        // the job's initial G was set on ctx->stream
+        HIP_TRY(hipEventRecord(s->created, s->ctx->stream));
+        HIP_TRY(hipStreamWaitEvent(L.st, s->created, 0));
     }
     r.job = job;
-    // slot: lowest index not used by another active request
+    // slot: lowest index not used by another request of this lane
     std::vector<char> busy(s->max_jobs, 0);
-    for (auto &a : s->active) busy[a->slot] = 1;
+    for (auto &a : L.active) busy[a->slot] = 1;
     int slot = 0;
     while (busy[slot]) slot++;
     r.slot = slot;
     // input: synthesised in HBM, reunitarized like the CLI does after loading
-    RC_TRY(gfcx::generate_smooth_device(job, r.seed, r.eps));
-    hipLaunchKernelGGL(gfc::k_reunit_aos, dim3(nblocks(job->V * 4)), dim3(kBlock), 0,
-                       s->ctx->stream, job->cfg, job->V * 4, s->ctx->flag_dev);
+    RC_TRY(gfcx::generate_smooth_device(job, L.st, r.seed, r.eps));
+    hipLaunchKernelGGL(gfc::k_reunit_aos, dim3(nblocks(job->V * 4)), dim3(kBlock), 0, L.st,
+                       job->cfg, job->V * 4, s->ctx->flag_dev);
     LAUNCH_CHECK();
-    RC_TRY(gfcx::build_u0(job));
-    RC_TRY(gfcx::set_identity(job));
+    RC_TRY(gfcx::build_u0(job, L.st));
+    RC_TRY(gfcx::set_identity(job, L.st));
     r.fp32 = r.o.precision == GFC_PREC_MIXED;
     r.state = INIT;
     r.t_admit = s->clock();
-    fill_desc(s, r);
+    fill_desc(L, r);
     admitted = true;
     s->stats.admitted++;
     return GFC_OK;
 }
 
-// Residues of every request in `which` (one batched launch + one sync).
-int evaluate(gfc_server *s, std::vector<Req *> &which) {
+// Residues of every request in `which` (one batched launch + one sync of L).
+int evaluate(gfc_server *s, Lane &L, std::vector<Req *> &which) {
     if (which.empty()) return GFC_OK;
     size_t nblk = 0;
     for (Req *r : which) nblk += nblocks(2 * (int64_t)r->job->Vh);
-    RC_TRY(grow_part(s, nblk));
-    RC_TRY(grow_table(s, s->d_oitems, s->h_oitems, s->oitems_cap, nblk));
-    HIP_TRY(hipStreamSynchronize(s->ctx->stream));  // pinned staging is reused
+    RC_TRY(grow_part(L, nblk));
+    RC_TRY(grow_table(L, L.d_oitems, L.h_oitems, L.oitems_cap, nblk));
+    HIP_TRY(hipStreamSynchronize(L.st));  // pinned staging is reused
     size_t k = 0;
     for (size_t q = 0; q < which.size(); q++) {
         Req *r = which[q];
-        s->h_jobs[r->slot].part_off = (int)k;
+        L.h_jobs[r->slot].part_off = (int)k;
         const int nb = (int)nblocks(2 * (int64_t)r->job->Vh);
         // the initial residue reads the FP64 G, like run_fix (engine.hip)
         const int g32 = r->state == RUN && r->fp32 ? 1 : 0;
-        for (int b = 0; b < nb; b++) s->h_oitems[k++] = make_int4(r->slot, b, g32, 0);
-        s->h_sel[q] = r->slot;
+        for (int b = 0; b < nb; b++) L.h_oitems[k++] = make_int4(r->slot, b, g32, 0);
+        L.h_sel[q] = r->slot;
     }
-    hipStream_t st = s->ctx->stream;
-    RC_TRY(upload_desc(s));
-    HIP_TRY(hipMemcpyAsync(s->d_oitems, s->h_oitems, k * sizeof(int4), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(s->d_sel, s->h_sel, which.size() * sizeof(int), hipMemcpyHostToDevice,
+    hipStream_t st = L.st;
+    RC_TRY(upload_desc(s, L));
+    HIP_TRY(hipMemcpyAsync(L.d_oitems, L.h_oitems, k * sizeof(int4), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(L.d_sel, L.h_sel, which.size() * sizeof(int), hipMemcpyHostToDevice,
                            st));
-    hipLaunchKernelGGL(gfc::k_obs_batch, dim3((unsigned)k), dim3(kBlock), 0, st, s->d_jobs,
-                       s->d_oitems, s->d_part);
+    hipLaunchKernelGGL(gfc::k_obs_batch, dim3((unsigned)k), dim3(kBlock), 0, st, L.d_jobs,
+                       L.d_oitems, L.d_part);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(gfc::k_reduce_batch, dim3((unsigned)which.size()), dim3(kBlock), 0, st,
-                       s->d_jobs, s->d_sel, s->d_part, s->d_out);
+                       L.d_jobs, L.d_sel, L.d_part, L.d_out);
     LAUNCH_CHECK();
-    HIP_TRY(hipMemcpyAsync(s->h_out, s->d_out, 3 * which.size() * sizeof(double),
+    HIP_TRY(hipMemcpyAsync(L.h_out, L.d_out, 3 * which.size() * sizeof(double),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     for (size_t q = 0; q < which.size(); q++) {
         Req *r = which[q];
         const double V = (double)r->job->V;
-        r->obs[0] = s->h_out[3 * q] / V;
-        r->obs[1] = s->h_out[3 * q + 1] / (3.0 * V);
-        r->obs[2] = s->h_out[3 * q + 2] / (9.0 * V);
+        r->obs[0] = L.h_out[3 * q] / V;
+        r->obs[1] = L.h_out[3 * q + 1] / (3.0 * V);
+        r->obs[2] = L.h_out[3 * q + 2] / (9.0 * V);
         r->evals++;
     }
     s->stats.residue_batches++;
     return GFC_OK;
 }
 
-int promote(gfc_server *s, Req &r) {
-    RC_TRY(gfcx::promote_g_only(r.job));
+int promote(Lane &L, Req &r) {
+    RC_TRY(gfcx::promote_g_only(r.job, L.st));
     r.fp32 = false;
-    fill_desc(s, r);
-    s->plan_sig.clear();
+    fill_desc(L, r);
+    L.plan_sig.clear();
     return GFC_OK;
 }
 
 // End of gaugefixOverrelaxation for r: hand-off, final reunitarization (unless
 // it stopped on max sweeps or a stalled residue), U = G U0 G^dagger.
-int finish(gfc_server *s, Req &r, int rc) {
-    if (r.fp32) RC_TRY(promote(s, r));
-    if (rc >= 0) RC_TRY(gfcx::reunit_g(r.job, false, 0, 2));
-    RC_TRY(gfcx::materialize(r.job));
+int finish(Lane &L, Req &r, int rc) {
+    if (r.fp32) RC_TRY(promote(L, r));
+    if (rc >= 0) RC_TRY(gfcx::reunit_g(r.job, L.st, false, 0, 2));
+    RC_TRY(gfcx::materialize(r.job, L.st));
     r.rc = rc >= 0 ? r.sweep : rc;
     r.state = DONE;
     return GFC_OK;
@@ -263,12 +300,13 @@ int finish(gfc_server *s, Req &r, int rc) {
 
 int next_multiple(int s, int m) { return (s / m + 1) * m; }
 
-int plan_and_run(gfc_server *s) {
-    // stop points: next reunitarization point, next check, next probe, max
+int plan_and_run(gfc_server *s, Lane &L) {
+    // stop points: next reunitarization point, next check, next probe,
+    // hand-off, max
     std::vector<Req *> run;
     std::vector<int> steps;
     int K = 0;
-    for (auto &a : s->active) {
+    for (auto &a : L.active) {
         if (a->state != RUN) continue;
         Req &r = *a;
         const int R = (int)r.p.sweeps_to_reunitarization;
@@ -294,8 +332,12 @@ int plan_and_run(gfc_server *s) {
         sig.push_back(steps[q]);
         sig.push_back(run[q]->sweep & 1);
     }
-    hipStream_t st = s->ctx->stream;
-    if (sig != s->plan_sig || s->graph == nullptr) {
+    hipStream_t st = L.st;
+    if (sig != L.plan_sig || L.graph[L.cur] == nullptr) {
+        // the other buffer: its plan's chunks have completed (the lane is
+        // served only once its older chunk is done, and the newer one runs
+        // the current plan)
+        const int b = L.cur ^ 1;
         // item table: for each step k, the 64-bit then the 32-bit runners
         std::vector<std::pair<size_t, size_t>> seg;  // per (step, precision): offset, count
         size_t total = 0;
@@ -304,8 +346,8 @@ int plan_and_run(gfc_server *s) {
                 for (size_t q = 0; q < run.size(); q++)
                     if (steps[q] > k && (int)run[q]->fp32 == prec)
                         total += nblocks(run[q]->job->Vh);
-        RC_TRY(grow_table(s, s->d_items, s->h_items, s->items_cap, total));
-        HIP_TRY(hipStreamSynchronize(st));
+        RC_TRY(grow_table(L, L.d_items[b], L.h_items[b], L.items_cap[b], total));
+        int4 *h_items = L.h_items[b], *d_items = L.d_items[b];
         size_t n = 0;
         for (int k = 0; k < K; k++)
             for (int prec = 0; prec < 2; prec++) {
@@ -315,17 +357,15 @@ int plan_and_run(gfc_server *s) {
                     if (steps[q] <= k || (int)r.fp32 != prec) continue;
                     const int par = (r.sweep + k) & 1;
                     for (int h0 = 0; h0 < r.job->Vh; h0 += kBlock)
-                        s->h_items[n++] = make_int4(r.slot, h0, par, 0);
+                        h_items[n++] = make_int4(r.slot, h0, par, 0);
                 }
                 seg.emplace_back(off, n - off);
             }
-        RC_TRY(upload_desc(s));
-        HIP_TRY(hipMemcpyAsync(s->d_items, s->h_items, n * sizeof(int4), hipMemcpyHostToDevice,
-                               st));
-        HIP_TRY(hipStreamSynchronize(st));
-        if (s->graph) {
-            hipGraphExecDestroy(s->graph);
-            s->graph = nullptr;
+        RC_TRY(upload_desc(s, L));
+        HIP_TRY(hipMemcpyAsync(d_items, h_items, n * sizeof(int4), hipMemcpyHostToDevice, st));
+        if (L.graph[b]) {
+            hipGraphExecDestroy(L.graph[b]);
+            L.graph[b] = nullptr;
         }
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -336,30 +376,31 @@ int plan_and_run(gfc_server *s) {
             const unsigned grid = (unsigned)((cnt + per - 1) / per);
             if (i % 2 == 0)
                 hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3(grid), dim3(kBlock), 0, st,
-                                   s->d_jobs, s->d_items + off, (int)cnt, per);
+                                   L.d_jobs, d_items + off, (int)cnt, per);
             else
                 hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3(grid), dim3(kBlock), 0, st,
-                                   s->d_jobs, s->d_items + off, (int)cnt, per);
+                                   L.d_jobs, d_items + off, (int)cnt, per);
         }
         hipError_t ce = hipStreamEndCapture(st, &g);
         if (ce != hipSuccess) return fail(GFC_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
-        ce = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+        ce = hipGraphInstantiate(&L.graph[b], g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (ce != hipSuccess) {
-            s->graph = nullptr;
+            L.graph[b] = nullptr;
             return fail(GFC_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ce));
         }
-        s->plan_sig = sig;
+        L.cur = b;
+        L.plan_sig = sig;
         s->stats.graphs_built++;
     } else {
         s->stats.graph_replays++;
     }
-    // at most two chunks in flight: new arrivals are admitted behind at most
-    // one queued chunk
-    hipEvent_t &ev = s->chunk_ev[s->stats.chunks % 2];
+    // at most two chunks of the lane in flight
+    hipEvent_t &ev = L.chunk_ev[L.chunks % 2];
     HIP_TRY(hipEventSynchronize(ev));
-    HIP_TRY(hipGraphLaunch(s->graph, st));
+    HIP_TRY(hipGraphLaunch(L.graph[L.cur], st));
     HIP_TRY(hipEventRecord(ev, st));
+    L.chunks++;
     s->stats.chunks++;
     for (size_t q = 0; q < run.size(); q++) {
         run[q]->sweep += steps[q];
@@ -396,31 +437,51 @@ int complete(gfc_server *s, std::unique_ptr<Req> &r) {
     return GFC_OK;
 }
 
-int poll_once(gfc_server *s) {
-    // 1. admission (FIFO, by footprint and slot)
-    while (!s->queue.empty()) {
+// Whether a lattice needs its residue at its current sweep.
+bool needs_residue(const Req &r) {
+    if (r.state == INIT) return true;
+    if (r.state != RUN) return false;
+    const bool check = r.sweep == r.due;
+    return check || (r.fp32 && !r.ho.at_floor && r.sweep % r.o.mixed_probe_every == 0);
+}
+
+// Whether its decisions at the current sweep need the lane idle: a residue,
+// or a change of its descriptor (hand-off, end on max sweeps), whose pinned
+// copy an upload still queued behind the chunk in flight may read.
+bool needs_idle(const Req &r) {
+    if (needs_residue(r)) return true;
+    return r.state == RUN && ((r.fp32 && r.sweep >= r.ho.switch_at) ||
+                              (int64_t)r.sweep >= (int64_t)r.p.max_sweeps_to_fix);
+}
+
+bool has_queued(const gfc_server *s, int lane) {
+    for (auto &q : s->queue)
+        if (q->lane == lane) return true;
+    return false;
+}
+
+// One poll of lane li (steps 1-4 of the header).
+int poll_lane(gfc_server *s, int li) {
+    Lane &L = s->lanes[li];
+    // 1. admission (FIFO within the size class, by footprint and slot)
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        if ((*it)->lane != li) {
+            ++it;
+            continue;
+        }
         bool ok = false;
-        RC_TRY(admit_one(s, *s->queue.front(), ok));
+        RC_TRY(admit_one(s, L, **it, ok));
         if (!ok) break;
-        s->active.push_back(std::move(s->queue.front()));
-        s->queue.pop_front();
+        L.active.push_back(std::move(*it));
+        it = s->queue.erase(it);
     }
     // 2. residues for starts, checks and probes
     std::vector<Req *> ev;
-    for (auto &a : s->active) {
-        Req &r = *a;
-        if (r.state == INIT) {
-            ev.push_back(&r);
-        } else if (r.state == RUN) {
-            const bool check = r.sweep == r.due;
-            const bool probe = r.fp32 && !r.ho.at_floor && !check &&
-                               r.sweep % r.o.mixed_probe_every == 0;
-            if (check || probe) ev.push_back(&r);
-        }
-    }
-    RC_TRY(evaluate(s, ev));
+    for (auto &a : L.active)
+        if (needs_residue(*a)) ev.push_back(a.get());
+    RC_TRY(evaluate(s, L, ev));
     // 3. decisions (the order of gaugefixOverrelaxation's loop body)
-    for (auto &a : s->active) {
+    for (auto &a : L.active) {
         Req &r = *a;
         const int which = r.p.gfix_proxy == GFC_PROXY_THETA ? 1 : 0;
         if (r.state == INIT) {
@@ -429,7 +490,7 @@ int poll_once(gfc_server *s) {
             r.due = gfc_next_check(0, r.res, &r.p);
             r.ho.start(r.res);
             r.state = RUN;
-            if (!(r.res > r.p.tolerance)) RC_TRY(finish(s, r, 0));
+            if (!(r.res > r.p.tolerance)) RC_TRY(finish(L, r, 0));
             continue;
         }
         if (r.state != RUN || r.sweep == 0) continue;
@@ -441,51 +502,96 @@ int poll_once(gfc_server *s) {
                                          s_ >= r.ho.switch_at));
         if (!at_stop) continue;
         if (r.fp32 && !r.ho.at_floor && s_ != r.due && s_ % r.o.mixed_probe_every == 0) {
-            if (r.ho.observe(r.o, r.p.tolerance, s_, r.obs[which], r.due)) RC_TRY(promote(s, r));
+            if (r.ho.observe(r.o, r.p.tolerance, s_, r.obs[which], r.due)) RC_TRY(promote(L, r));
         }
         if (s_ == r.due) {
             const double last = r.res;
             r.res = r.obs[which];
             r.F = r.obs[2];
             if (std::fabs(r.res - last) / last < 10E-16) {
-                RC_TRY(finish(s, r, GFC_ENOTCONVERGED));
+                RC_TRY(finish(L, r, GFC_ENOTCONVERGED));
                 continue;
             } else if (r.res <= r.p.tolerance) {
-                RC_TRY(finish(s, r, 0));
+                RC_TRY(finish(L, r, 0));
                 continue;
             } else {
                 r.due = gfc_next_check((unsigned)s_, r.res, &r.p);
                 if (r.fp32 && r.ho.observe(r.o, r.p.tolerance, s_, r.res, r.due))
-                    RC_TRY(promote(s, r));
+                    RC_TRY(promote(L, r));
             }
         }
-        if (r.fp32 && s_ >= r.ho.switch_at) RC_TRY(promote(s, r));
+        if (r.fp32 && s_ >= r.ho.switch_at) RC_TRY(promote(L, r));
         if ((unsigned)s_ >= r.p.max_sweeps_to_fix) {
-            RC_TRY(finish(s, r, GFC_ENOTCONVERGED));
+            RC_TRY(finish(L, r, GFC_ENOTCONVERGED));
             continue;
         }
-        if (!((unsigned)s_ % r.p.sweeps_to_reunitarization)) RC_TRY(gfcx::reunit_g(r.job, r.fp32, 0, 2));
+        if (!((unsigned)s_ % r.p.sweeps_to_reunitarization))
+            RC_TRY(gfcx::reunit_g(r.job, L.st, r.fp32, 0, 2));
     }
     // retire finished requests (their results are materialised in HBM)
     bool any_done = false;
-    for (auto &a : s->active) any_done |= a->state == DONE;
+    for (auto &a : L.active) any_done |= a->state == DONE;
     if (any_done) {
-        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(L.st));
         const double t = s->clock();
-        for (auto &a : s->active)
+        for (auto &a : L.active)
             if (a->state == DONE) {
                 a->t_done = t;
                 RC_TRY(complete(s, a));
             }
-        s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
-                                       [](const std::unique_ptr<Req> &a) {
-                                           return !a || a->state == DONE || a->job == nullptr;
-                                       }),
-                        s->active.end());
-        s->plan_sig.clear();
+        L.active.erase(std::remove_if(L.active.begin(), L.active.end(),
+                                      [](const std::unique_ptr<Req> &a) {
+                                          return !a || a->state == DONE || a->job == nullptr;
+                                      }),
+                       L.active.end());
+        L.plan_sig.clear();
     }
     // 4. one chunk of packed half-sweeps
-    return plan_and_run(s);
+    return plan_and_run(s, L);
+}
+
+// Serves every lane that can go on without waiting: one with queued requests
+// or a lattice that needs the lane idle waits for all its chunks, any other
+// one for its older chunk.  Sleeps a moment when none can.
+int poll_once(gfc_server *s) {
+    bool served = false;
+    for (int li = 0; li < kLanes; li++) {
+        Lane &L = s->lanes[li];
+        const bool queued = has_queued(s, li);
+        if (L.active.empty() && !queued) continue;
+        bool sync = queued;
+        for (auto &a : L.active) sync |= needs_idle(*a);
+        // chunk_ev[chunks % 2] marks the older chunk in flight, the other the newer
+        hipEvent_t e = L.chunk_ev[(L.chunks + (sync ? 1 : 0)) % 2];
+        const hipError_t q = hipEventQuery(e);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return fail(GFC_EDEVICE, hipGetErrorString(q));
+        RC_TRY(poll_lane(s, li));
+        served = true;
+    }
+    if (!served) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    return GFC_OK;
+}
+
+void free_lane(Lane &L) {
+    if (L.st) hipStreamSynchronize(L.st);
+    for (int b = 0; b < 2; b++) {
+        if (L.graph[b]) hipGraphExecDestroy(L.graph[b]);
+        hipFree(L.d_items[b]);
+        hipHostFree(L.h_items[b]);
+    }
+    for (hipEvent_t e : L.chunk_ev)
+        if (e) hipEventDestroy(e);
+    for (auto &a : L.active) gfc_job_destroy(a->job);
+    hipFree(L.d_jobs);
+    hipFree(L.d_oitems);
+    hipFree(L.d_sel);
+    hipFree(L.d_part);
+    hipFree(L.d_out);
+    hipHostFree(L.h_jobs);
+    hipHostFree(L.h_oitems);
+    hipHostFree(L.h_sel);
+    hipHostFree(L.h_out);
 }
 
 }  // namespace
@@ -498,7 +604,7 @@ int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_r
         return fail(GFC_EINVAL, "gfc_server_create: bad argument");
     *out = nullptr;
     HIP_TRY(hipSetDevice(ctx->device));
-    auto s = std::make_unique<gfc_server>();
+    std::unique_ptr<gfc_server, void (*)(gfc_server *)> s(new gfc_server(), gfc_server_destroy);
     s->ctx = ctx;
     s->max_jobs = max_jobs;
     s->keep = keep_results;
@@ -509,18 +615,23 @@ int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_r
         mem_budget = (int64_t)(0.85 * (double)fr);
     }
     s->budget = mem_budget;
-    s->d_jobs = dev_alloc<BatchJob>(max_jobs);
-    s->d_sel = dev_alloc<int>(max_jobs);
-    s->d_out = dev_alloc<double>(3 * (size_t)max_jobs);
-    if (!s->d_jobs || !s->d_sel || !s->d_out) return fail(GFC_ENOMEM, "gfc_server_create");
-    HIP_TRY(hipHostMalloc(&s->h_jobs, sizeof(BatchJob) * max_jobs, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc(&s->h_sel, sizeof(int) * max_jobs, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc(&s->h_out, sizeof(double) * 3 * max_jobs, hipHostMallocDefault));
-    std::memset(s->h_jobs, 0, sizeof(BatchJob) * max_jobs);
-    HIP_TRY(hipEventCreateWithFlags(&s->chunk_ev[0], hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&s->chunk_ev[1], hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(s->chunk_ev[0], ctx->stream));
-    HIP_TRY(hipEventRecord(s->chunk_ev[1], ctx->stream));
+    HIP_TRY(hipEventCreateWithFlags(&s->created, hipEventDisableTiming));
+    for (int li = 0; li < kLanes; li++) {
+        Lane &L = s->lanes[li];
+        L.st = ctx->lane[li];
+        L.d_jobs = dev_alloc<BatchJob>(max_jobs);
+        L.d_sel = dev_alloc<int>(max_jobs);
+        L.d_out = dev_alloc<double>(3 * (size_t)max_jobs);
+        if (!L.d_jobs || !L.d_sel || !L.d_out) return fail(GFC_ENOMEM, "gfc_server_create");
+        HIP_TRY(hipHostMalloc(&L.h_jobs, sizeof(BatchJob) * max_jobs, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&L.h_sel, sizeof(int) * max_jobs, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&L.h_out, sizeof(double) * 3 * max_jobs, hipHostMallocDefault));
+        std::memset(L.h_jobs, 0, sizeof(BatchJob) * max_jobs);
+        for (int k = 0; k < 2; k++) {
+            HIP_TRY(hipEventCreateWithFlags(&L.chunk_ev[k], hipEventDisableTiming));
+            HIP_TRY(hipEventRecord(L.chunk_ev[k], L.st));
+        }
+    }
     *out = s.release();
     return GFC_OK;
 }
@@ -528,24 +639,11 @@ int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_r
 void gfc_server_destroy(gfc_server *s) {
     if (s == nullptr) return;
     hipSetDevice(s->ctx->device);
+    for (Lane &L : s->lanes) free_lane(L);
     hipStreamSynchronize(s->ctx->stream);
-    if (s->graph) hipGraphExecDestroy(s->graph);
-    for (hipEvent_t e : s->chunk_ev)
-        if (e) hipEventDestroy(e);
-    for (auto &a : s->active) gfc_job_destroy(a->job);
     for (auto &kv : s->kept) gfc_job_destroy(kv.second->job);
     for (auto &kv : s->cache) gfc_job_destroy(kv.second);
-    hipFree(s->d_jobs);
-    hipFree(s->d_items);
-    hipFree(s->d_oitems);
-    hipFree(s->d_sel);
-    hipFree(s->d_part);
-    hipFree(s->d_out);
-    hipHostFree(s->h_jobs);
-    hipHostFree(s->h_items);
-    hipHostFree(s->h_oitems);
-    hipHostFree(s->h_sel);
-    hipHostFree(s->h_out);
+    if (s->created) hipEventDestroy(s->created);
     delete s;
 }
 
@@ -570,6 +668,7 @@ int64_t gfc_server_submit(gfc_server *s, const gfc_geom *g, uint64_t seed, doubl
         gfc_options_default(&r->o);
     if (r->o.mixed_probe_every <= 0) r->o.mixed_probe_every = 50;
     r->prec = r->o.precision == GFC_PREC_MIXED ? GFC_PREC_MIXED : GFC_PREC_FP64;
+    r->lane = lane_of(r->geom);
     r->footprint = footprint_of(*r);
     if (r->footprint > s->budget)
         return fail(GFC_ENOMEM, "gfc_server_submit: lattice larger than the memory budget");
@@ -592,14 +691,14 @@ int gfc_server_poll(gfc_server *s, gfc_completion *out, int max_out) {
 }
 
 int gfc_server_pending(gfc_server *s) {
-    return s ? (int)(s->queue.size() + s->active.size() + s->done.size()) : 0;
+    return s ? (int)(s->queue.size() + s->n_active() + s->done.size()) : 0;
 }
 
 int gfc_server_get_stats(gfc_server *s, gfc_server_stats *st) {
     if (s == nullptr || st == nullptr) return fail(GFC_EINVAL, "gfc_server_get_stats: NULL");
     *st = s->stats;
     st->queued = (int)s->queue.size();
-    st->active = (int)s->active.size();
+    st->active = s->n_active();
     st->bytes_used = s->used;
     st->bytes_budget = s->budget;
     return GFC_OK;

@@ -89,3 +89,33 @@ def test_short_stream(ctx):
         summary, comps = serve.run_stream(srv, reqs)
     assert summary["all_converged"]
     assert summary["p99_ms"] >= summary["p50_ms"] > 0
+
+
+def test_every_lane_alone_equals_in_the_mix(ctx):
+    """One lattice per size lane (each lane has its own stream and chunk
+    plans), two of the small one: each result equals its lone run bit for
+    bit, in mixed precision with the deferred hand-off."""
+    reqs = [(gfc.Geometry(8, 16), 21, 0.3), (gfc.Geometry(24, 24), 22, 0.3),
+            (gfc.Geometry(32, 32), 23, 0.3), (gfc.Geometry(48, 48), 24, 0.3),
+            (gfc.Geometry(8, 16), 25, 0.35)]
+    opts = gfc.Options(precision=gfc.PREC_MIXED)
+    p = gfc.Params(tolerance=1e-16, sweeps_to_reunitarization=10)
+    with gfc.Server(ctx, max_jobs=8, keep_results=True) as srv:
+        for k, (geom, seed, eps) in enumerate(reqs):
+            srv.submit(geom, seed, eps, p, opts, tag=k)
+        comps = srv.drain(timeout=300)
+        assert len(comps) == len(reqs)
+        # the small lattices finish long before the large one
+        t = {c.tag: c.t_done for c in comps}
+        assert t[0] < t[3] and t[4] < t[3]
+        for c in comps:
+            geom, seed, eps = reqs[c.tag]
+            rc, rep, G_alone = _alone(ctx, geom, seed, eps, p, opts)
+            print(f"{geom.ns}^3x{geom.nt}: sweeps {c.sweeps} fp32 {c.sweeps_fp32} "
+                  f"latency {(c.t_done - c.t_submit) * 1e3:.1f} ms")
+            assert c.status == rc and c.sweeps == rep.sweeps
+            assert c.sweeps_fp32 == rep.sweeps_fp32
+            assert c.residue_final == rep.residue_final
+            G_mix = srv.result_gauge(c.id, geom)
+            assert np.array_equal(G_mix, G_alone), np.max(np.abs(G_mix - G_alone))
+            srv.release(c.id)
