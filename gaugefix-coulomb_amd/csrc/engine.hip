@@ -625,10 +625,10 @@ struct SliceTracker {
         const int n = n_items[p];
         if (fp32)
             hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3(n), dim3(kBlock), 0, s, d_job,
-                               d_items[p], n, 1);
+                               d_items[p], n);
         else
             hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3(n), dim3(kBlock), 0, s, d_job,
-                               d_items[p], n, 1);
+                               d_items[p], n);
         LAUNCH_CHECK();
         return GFC_OK;
     }

@@ -640,21 +640,20 @@ struct BatchJob {
     int part_off;     // first partial-sum slot of this lattice (observables)
 };
 
-// item = (job index, first parity index, parity, unused)
+// One workgroup per item = (job index, first parity index, parity, unused).
+// (Looping a workgroup over several items measured 17% slower.)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sweep_batch(const BatchJob *__restrict__ jobs,
                                                         const int4 *__restrict__ items,
-                                                        int nitems, int per_block) {
-    const int b = (int)blockIdx.x;
-    const int i1 = min((b + 1) * per_block, nitems);
-    for (int i = b * per_block; i < i1; i++) {
-        const int4 it = items[i];
-        const BatchJob J = jobs[it.x];
-        const int h = it.y + (int)threadIdx.x;
-        if (h < J.Vh)
-            sweep_site<T>(static_cast<const cplx<T> *>(J.u0), static_cast<cplx<T> *>(J.g), J.ns,
-                          J.Vh, it.z, (T)J.omega, J.hits, h);
-    }
+                                                        int nitems) {
+    const int i = (int)blockIdx.x;
+    if (i >= nitems) return;
+    const int4 it = items[i];
+    const BatchJob J = jobs[it.x];
+    const int h = it.y + (int)threadIdx.x;
+    if (h < J.Vh)
+        sweep_site<T>(static_cast<const cplx<T> *>(J.u0), static_cast<cplx<T> *>(J.g), J.ns,
+                      J.Vh, it.z, (T)J.omega, J.hits, h);
 }
 
 // item = (job index, block index within the lattice's [0, 2 Vh) range, fp32 G?, _)

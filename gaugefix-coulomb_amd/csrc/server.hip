@@ -372,14 +372,12 @@ int plan_and_run(gfc_server *s, Lane &L) {
         for (size_t i = 0; i < seg.size(); i++) {
             const size_t off = seg[i].first, cnt = seg[i].second;
             if (cnt == 0) continue;
-            const int per = (int)std::max<size_t>(1, cnt / 4096);
-            const unsigned grid = (unsigned)((cnt + per - 1) / per);
             if (i % 2 == 0)
-                hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3(grid), dim3(kBlock), 0, st,
-                                   L.d_jobs, d_items + off, (int)cnt, per);
+                hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3((unsigned)cnt), dim3(kBlock),
+                                   0, st, L.d_jobs, d_items + off, (int)cnt);
             else
-                hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3(grid), dim3(kBlock), 0, st,
-                                   L.d_jobs, d_items + off, (int)cnt, per);
+                hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3((unsigned)cnt), dim3(kBlock),
+                                   0, st, L.d_jobs, d_items + off, (int)cnt);
         }
         hipError_t ce = hipStreamEndCapture(st, &g);
         if (ce != hipSuccess) return fail(GFC_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
