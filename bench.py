@@ -392,7 +392,11 @@ def main():
             f"fp32 {dmx.sweeps_fp32}) speedup {d64_s / dmx_s:.3f}")
     io = None
     if not a.no_io:
-        io = io_leg(job, geom, U)
+        try:
+            io = io_leg(job, geom, U)
+        except Exception as e:  # e.g. no room for the 3 GB file: report, go on
+            io = {"error": f"{type(e).__name__}: {e}"}
+            log(f"io leg skipped: {io['error']}")
     job.close()
 
     serve_out = None

@@ -366,6 +366,26 @@ int build_u0(gfc_job *j, hipStream_t s) {
     return GFC_OK;
 }
 
+// u0 from cfg and G <- 1 at the start of a fix; the FP32 copies only for a
+// mixed-precision fix.
+int prepare_fix(gfc_job *j, bool mixed) {
+    hipStream_t s = j->ctx->stream;
+    const unsigned b = nblocks(2 * (int64_t)j->Vh);
+    hipLaunchKernelGGL(gfc::k_build_u0<double>, dim3(b), dim3(kBlock), 0, s, j->cfg, j->u0,
+                       j->geom.ns, j->Vh);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(gfc::k_g_identity<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh);
+    LAUNCH_CHECK();
+    if (mixed) {
+        hipLaunchKernelGGL(gfc::k_build_u0<float>, dim3(b), dim3(kBlock), 0, s, j->cfg, j->u0f,
+                           j->geom.ns, j->Vh);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(gfc::k_g_identity<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh);
+        LAUNCH_CHECK();
+    }
+    return GFC_OK;
+}
+
 int materialize(gfc_job *j) { return materialize(j, j->ctx->stream); }
 
 int materialize(gfc_job *j, hipStream_t s) {
@@ -685,8 +705,7 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     gfc_report r;
     std::memset(&r, 0, sizeof r);
 
-    RC_TRY(build_u0(j));
-    RC_TRY(set_identity(j));
+    RC_TRY(prepare_fix(j, o.precision == GFC_PREC_MIXED));
     const double omega = (double)p->omega_OR;  // float in the reference
     const int hits = (int)p->hits;
     const int which = p->gfix_proxy == GFC_PROXY_THETA ? 1 : 0;
