@@ -5,13 +5,13 @@
 //   u0  : SoA spatial links of the configuration being fixed
 //   g   : SoA gauge transformation
 // u0 and g hold SU(3) matrices as their first two rows (6 complex numbers);
-// row 2 = conj(r0 x r1) is rebuilt in registers on every load.  One plane of Vh
-// complex values per (parity p, [direction i = Z,Y,X,] element k < 6):
-//   u0 plane (p*3 + i)*6 + k,   g plane p*6 + k.
+// row 2 = conj(r0 x r1) is rebuilt in registers on every load.  One tiled field
+// (see field_size below) per parity p and, for u0, direction i = Z,Y,X:
+//   u0 field p*3 + i,   g field p.
 // Within a parity the index is h = lex >> 1 with lex = ((t ns + z) ns + y) ns + x
 // (ns even), so the y/z neighbours of a site sit at a constant offset +-ns/2,
-// +-ns^2/2 in the other parity's planes and the x neighbours at h or h+-1:
-// every wavefront reads 64 consecutive complex values per plane.
+// +-ns^2/2 in the other parity's field and the x neighbours at h or h+-1; a
+// timeslice t is the range [t ns^3/2, (t+1) ns^3/2) of h.
 //
 // The sweep keeps the reference's state in factored form: the current links are
 // U_i(n) = G(n) u0_i(n) G(n+i)^dagger, so a half-sweep only rewrites G on the
@@ -30,12 +30,6 @@ constexpr int kPlanes = 6;  // stored complex elements per SU(3) matrix
 // Kernels and device helpers have internal linkage: each translation unit
 // that launches them (engine.hip, server.hip) carries its own code object.
 namespace {
-
-// Workgroups are dispatched round-robin over the 8 XCDs and keep that order:
-// the workgroups running at any moment cover one contiguous stretch of sites,
-// which the sweep kernels stream through together.  (Remapping each XCD onto
-// its own contiguous range, so that z-neighbour planes share one L2, measured
-// 5% slower at 48^3 x 96.)
 
 // Loads that stream (read once per sweep) carry the non-temporal hint so that
 // the L2 keeps the neighbour G matrices, which six sites read.
@@ -206,6 +200,11 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
 }
 
 // One red/black half-sweep on parity p over parity indices [h0, h0 + count).
+// Workgroups are dispatched round-robin over the 8 XCDs and keep that order:
+// the workgroups running at any moment cover one contiguous stretch of sites,
+// which they stream through together.  (Remapping each XCD onto its own
+// contiguous range, so that z-neighbour G share one L2, measured 5% slower at
+// 48^3 x 96; runs of 4-32 workgroups per XCD measured the same as plain order.)
 template <typename T, bool POUT = false>
 __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0,
                                                   cplx<T> *__restrict__ g, int ns, int Vh,
