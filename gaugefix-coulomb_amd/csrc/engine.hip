@@ -1,11 +1,13 @@
 // engine.hip -- C ABI of the Coulomb gauge-fixing engine (see include/gfc.h).
 //
-// One gfc_ctx owns a HIP stream on one device; one gfc_job owns the device
-// fields of one lattice.  The driver loop of gfc_job_fix restates
+// One gfc_ctx owns four HIP streams on one device; one gfc_job owns the
+// device fields of one lattice.  The driver loop of gfc_job_fix restates
 // gaugefixOverrelaxation (src/gauge_fixing.c:713-826 of the reference):
 // G <- 1, residue, check schedule, half-sweeps with alternating parity,
 // stall / convergence / max-sweeps tests, reunitarization every
-// sweeps_to_reunitarization sweeps and once at the end.
+// sweeps_to_reunitarization sweeps and once at the end.  Between two host
+// decisions the half-sweeps run timeslice group by timeslice group on the
+// four streams (sweep_blocked), bit-identical to whole-lattice sweeps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

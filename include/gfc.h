@@ -115,8 +115,11 @@ void gfc_field_to_fp32(const double *in, float *out, int64_t n_matrices);
 
 /* ---- device engine ------------------------------------------------------ */
 #define GFC_PREC_FP64 0  /* every sweep in FP64 (drop-in)                      */
-#define GFC_PREC_MIXED 1 /* FP32 sweeps until the residue reaches
-                            mixed_switch, then FP64 sweeps to the tolerance   */
+#define GFC_PREC_MIXED 1 /* FP32 sweeps until the FP32 phase is finished
+                            (mixed_switch or its rounding floor) and, with
+                            mixed_defer, until the FP64 sweeps left before
+                            the next check suffice; then FP64 sweeps to the
+                            tolerance                                          */
 
 typedef struct {
     int precision;          /* GFC_PREC_FP64 (default) or GFC_PREC_MIXED      */
@@ -288,9 +291,13 @@ typedef struct {
     int64_t bytes_used, bytes_budget;
 } gfc_server_stats;
 
-/* A scheduler on ctx's stream.  mem_budget (bytes, <= 0: 85% of the free HBM)
- * bounds the summed footprints of resident lattices; max_jobs bounds their
- * number; keep_results keeps finished lattices resident until released. */
+/* A scheduler on ctx's four streams: lattices are served in four lanes by
+ * size (fewer than 2^18, 2^20, 2^22 sites, and larger), one stream each, so
+ * that small lattices do not advance at the pace of large ones.  mem_budget
+ * (bytes, <= 0: 85% of the free HBM) bounds the summed footprints of resident
+ * lattices; max_jobs bounds their number; keep_results keeps finished
+ * lattices resident until released.  gfc_server_poll never waits for a busy
+ * lane (it sleeps 20 us when no lane can proceed). */
 int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_results,
                       gfc_server **server);
 void gfc_server_destroy(gfc_server *server);
