@@ -134,8 +134,10 @@ typedef struct {
                                FP32 phase's convergence rate (x1.1 + 10);
                                0: hand over at once                           */
     int verbose;            /* 1: print "Sweeps: ..." lines like the reference */
-    int time_sweeps;        /* 1: bracket every sweep kernel with HIP events on
-                               the job's stream and report their total in
+    int time_sweeps;        /* 1: bracket every whole-lattice sweep launch, or
+                               every timeslice-blocked run of sweeps between
+                               two host decisions, with HIP events on the job's
+                               stream and report their total in
                                gfc_report.seconds_sweeps                       */
     int retire_slices;      /* 1: per-timeslice convergence tracking.  Every
                                mixed_probe_every sweeps (both precisions) the
@@ -173,7 +175,7 @@ typedef struct {
 
 void gfc_options_default(gfc_options *o);
 
-typedef struct gfc_ctx gfc_ctx; /* one HIP device + stream */
+typedef struct gfc_ctx gfc_ctx; /* one HIP device + four streams */
 typedef struct gfc_job gfc_job; /* one lattice resident in HBM */
 
 const char *gfc_last_error(void);
