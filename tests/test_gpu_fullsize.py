@@ -1,5 +1,9 @@
 """Full-size runs (48^3 x 96 from FP32 input; 64^3 x 128 with per-timeslice
-retirement) checked through properties that hold at any size:
+retirement).  The 48^3 x 96 run is compared with the compiled reference's
+own run on the same input (tests/golden/ref_48x96_fix.json, written by
+tests/golden/make_ref_fullsize.py): same sweep count, F to 1e-12 (FP64) and
+1e-10 (mixed) relative, plaquettes to 1e-14.  Both runs are also checked
+through properties that hold at any size:
 
 * the fix stops at the first scheduled check, sweep gfc_next_check(0, e2(0)),
   with e2 <= 1e-16 there;
@@ -9,6 +13,9 @@ retirement) checked through properties that hold at any size:
 * the gauge functional F rises, and the mixed-precision result reaches the
   same gauge as the FP64 one (F to 1e-10 relative).
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -17,6 +24,7 @@ import gfc
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
 RNG = np.random.default_rng(7)
+REF_48 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_48x96_fix.json")
 
 
 def _neighbour(site, mu, ns, nt):
@@ -63,7 +71,11 @@ def test_48x96_fp32_input_fp64_and_mixed(ctx):
         job.save_config()
         samples = _samples(job, geom)
         plaq0 = job.plaquette()
-        F0 = job.observables()[2]
+        e2_0, _, F0 = job.observables()
+        ref = json.load(open(REF_48))
+        assert abs(plaq0[0] - ref["plaq0"][0]) < 1e-14 and abs(plaq0[1] - ref["plaq0"][1]) < 1e-14
+        assert abs(e2_0 - ref["e2_0"]) / ref["e2_0"] < 1e-12
+        assert abs(F0 - ref["F0"]) < 1e-15
         Fs = {}
         for prec in (gfc.PREC_FP64, gfc.PREC_MIXED):
             job.restore_config()
@@ -71,8 +83,14 @@ def test_48x96_fp32_input_fp64_and_mixed(ctx):
             first = p.next_check(0, rep.residue_initial)
             print(f"48^3x96 precision {prec}: {rc} sweeps (first check {first}), "
                   f"fp32 {rep.sweeps_fp32}, e2 {rep.residue_final:.3e}, {rep.seconds:.3f} s")
-            assert rc == first
+            assert rc == first == ref["sweeps"]
             Fs[prec] = _check_fixed(job, geom, samples, plaq0, p, rep.residue_initial)
+            plaq = job.plaquette()
+            rel = abs(Fs[prec] - ref["F"]) / ref["F"]
+            print(f"  vs reference: F rel diff {rel:.2e}, plaquette diffs "
+                  f"{abs(plaq[0] - ref['plaq'][0]):.1e} {abs(plaq[1] - ref['plaq'][1]):.1e}")
+            assert rel < (1e-12 if prec == gfc.PREC_FP64 else 1e-10)
+            assert abs(plaq[0] - ref["plaq"][0]) < 1e-14 and abs(plaq[1] - ref["plaq"][1]) < 1e-14
         assert Fs[gfc.PREC_FP64] > F0
         assert abs(Fs[gfc.PREC_MIXED] - Fs[gfc.PREC_FP64]) / Fs[gfc.PREC_FP64] < 1e-10
 
