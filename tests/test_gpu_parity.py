@@ -8,6 +8,7 @@ seeded input.  Tolerances are those of the FP64 drop-in contract:
   convergence within +-1% (exact here).
 """
 import hashlib
+import os
 
 import numpy as np
 import pytest
@@ -134,12 +135,19 @@ def test_drop_in_host_arrays(ctx):
     assert np.max(np.abs(U - f["U_200"])) < 1e-12
 
 
-def test_config2_32x64(ctx):
-    """Config 2: 32^3x64, eps 0.3 -- e2 after k sweeps, G after 200, plaquette
+def test_config2_32x64(ctx, tmp_path):
+    """Config 2: 32^3x64, eps 0.3, read from a little-endian FP64 SCIDAC file
+    (streamed into HBM) -- e2 after k sweeps, G after 200, plaquette
     invariance and the sweep count to 1e-16 against the reference."""
     f = golden("lat_32x64.npz")
     geom, U = _config(f)
-    job = _job(ctx, geom, U)
+    path = str(tmp_path / "cfg_32x64_le64.bin")
+    assert gfc.write_config(path, geom, U, 8, False) == 0
+    job = gfc.Job(ctx, geom)
+    assert job.load_config(path, 8, byte_swap=False) == 0
+    os.remove(path)
+    assert job.reunitarize() == 0
+    job.save_config()
     p0 = job.plaquette()
     assert np.allclose(p0, f["obs_reunit"][3:], atol=1e-14)
     for i, k in enumerate(f["traj_k"]):
@@ -155,8 +163,7 @@ def test_config2_32x64(ctx):
             dG = np.max(np.abs(G[f["G_sites"]] - f["G_200"]))
             print(f"G after 200: max|dG| = {dG:.3e}")
             assert dG < G_ABS
-        job.upload_config(U)
-        job.reunitarize()
+        job.restore_config()
     rc, rep = job.fix(gfc.Params(sweeps_to_reunitarization=10))
     ref_rc = int(f["full_est1000_rc"])
     print(f"sweeps {rc} reference {ref_rc}")
