@@ -20,6 +20,8 @@ reference's code.  Outputs:
                    KEEP_G, U after 200, runs to tolerance (estimate 1000 and 0)
   lat_8x16.npz     config 1 (8^3x16, eps 0.25, seed 2): observables after
                    k = 1..200, G (sampled sites) after 200, runs to tolerance
+  lat_2x2.npz      2^3x2 (eps 0.3, seed 11) and 6^3x4 (eps 0.3, seed 12),
+  lat_6x4.npz      the same records as lat_4x8.npz
   lat_32x64.npz    config 2 (32^3x64, eps 0.3, seed 3): observables after
                    k in K_LARGE, G (sampled sites) after 200, run to tolerance
   io.npz           loadConfig results (SHA-256 of the loaded arrays and return
@@ -241,6 +243,12 @@ def main():
         if not only or "8x16" in only:
             lattice(tmp, "lat_8x16", 8, 16, 0.25, 2, range(1, 201), [200], "ref_golden_le64",
                     keep_u=())
+        # the smallest lattice (the +mu and -mu neighbours coincide) and a
+        # spatial extent that is not a multiple of 4
+        if not only or "2x2" in only:
+            lattice(tmp, "lat_2x2", 2, 2, 0.3, 11, range(1, 201), KEEP_G, "ref_golden_le64")
+        if not only or "6x4" in only:
+            lattice(tmp, "lat_6x4", 6, 4, 0.3, 12, range(1, 201), KEEP_G, "ref_golden_le64")
         if (not only and not a.skip_large) or "32x64" in only:
             lattice(tmp, "lat_32x64", 32, 64, 0.3, 3, K_LARGE, [200], "ref_golden_le64_mt",
                     full_runs=((1000, 1e-16),), keep_u=())

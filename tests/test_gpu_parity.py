@@ -58,16 +58,20 @@ def test_reunitarize_and_observables_4x8(ctx):
     job.close()
 
 
+SMALL = ["lat_2x2.npz", "lat_4x8.npz", "lat_6x4.npz"]
+
+
+@pytest.mark.parametrize("name", SMALL)
 @pytest.mark.parametrize("k", [1, 2, 3, 10, 11, 200])
-def test_state_after_k_sweeps_4x8(ctx, k):
-    f = golden("lat_4x8.npz")
+def test_state_after_k_sweeps(ctx, name, k):
+    f = golden(name)
     geom, U = _config(f)
     job = _job(ctx, geom, U)
     rc, rep = job.fix(_fixed_k(k))
     assert rc == gfc.ENOTCONVERGED and rep.sweeps == k
     G = job.download_gauge()
     dG = np.max(np.abs(G - f[f"G_{k}"]))
-    print(f"k={k} max|dG|={dG:.3e}")
+    print(f"{name} k={k} max|dG|={dG:.3e}")
     assert dG < G_ABS
     i = list(f["traj_k"]).index(k)
     e2, th, F = job.observables()
@@ -83,7 +87,7 @@ def test_state_after_k_sweeps_4x8(ctx, k):
     job.close()
 
 
-@pytest.mark.parametrize("name", ["lat_4x8.npz", "lat_8x16.npz"])
+@pytest.mark.parametrize("name", SMALL + ["lat_8x16.npz"])
 def test_e2_every_sweep(ctx, name):
     f = golden(name)
     geom, U = _config(f)
@@ -98,8 +102,8 @@ def test_e2_every_sweep(ctx, name):
     job.close()
 
 
-@pytest.mark.parametrize("name,est", [("lat_4x8.npz", 1000), ("lat_4x8.npz", 0),
-                                      ("lat_8x16.npz", 1000), ("lat_8x16.npz", 0)])
+@pytest.mark.parametrize("name", SMALL + ["lat_8x16.npz"])
+@pytest.mark.parametrize("est", [1000, 0])
 def test_full_run_to_tolerance(ctx, name, est):
     f = golden(name)
     geom, U = _config(f)

@@ -65,7 +65,10 @@ def _obs(U, g):
                      orc.plaquette(U, g, 0), orc.plaquette(U, g, 1)])
 
 
-@pytest.mark.parametrize("name", ["lat_4x8.npz", "lat_8x16.npz"])
+SMALL = ["lat_2x2.npz", "lat_4x8.npz", "lat_6x4.npz"]
+
+
+@pytest.mark.parametrize("name", SMALL + ["lat_8x16.npz"])
 def test_observables_and_reunitarization(name):
     f = golden(name)
     geom, U = _config(f)
@@ -77,9 +80,10 @@ def test_observables_and_reunitarization(name):
     assert np.allclose(_obs(U, g), f["obs_reunit"], rtol=REL, atol=1e-15)
 
 
-def test_trajectory_4x8_all_sweeps():
+@pytest.mark.parametrize("name", SMALL)
+def test_trajectory_all_sweeps(name):
     """e2 after every half-sweep k = 1..200 (reference: one run per k)."""
-    f = golden("lat_4x8.npz")
+    f = golden(name)
     geom, U = _config(f)
     g = orc.geom(geom.ns, geom.nt)
     orc.reunitarize(U)
@@ -110,9 +114,10 @@ def test_state_after_k_sweeps_4x8(k):
     assert np.allclose(_obs(U, g), f["traj_obs"][i], rtol=1e-13)
 
 
+@pytest.mark.parametrize("name", SMALL)
 @pytest.mark.parametrize("est", [1000, 0])
-def test_full_run_sweep_count_4x8(est):
-    f = golden("lat_4x8.npz")
+def test_full_run_sweep_count(name, est):
+    f = golden(name)
     geom, U = _config(f)
     g = orc.geom(geom.ns, geom.nt)
     orc.reunitarize(U)
