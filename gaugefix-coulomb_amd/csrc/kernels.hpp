@@ -169,8 +169,11 @@ __device__ __forceinline__ Nbr neighbours(const Site &s, int h, int ns) {
 // batched (serving) kernels, so a site computes the same bits in both.
 // POUT projects the new G(n) onto SU(3) before storing it: the same bits as a
 // reunitarization of this parity's G right after the sweep, since the
-// projection reads rows 0 and 1 only.
-template <typename T, bool POUT = false>
+// projection reads rows 0 and 1 only.  STREAM reads u0 and G(n) with
+// non-temporal loads (data read once per sweep); without it they are cached,
+// for sweeps whose data stays in the Infinity Cache from one half-sweep to the
+// next.
+template <typename T, bool POUT = false, bool STREAM = true>
 __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
                                            cplx<T> *__restrict__ g, int ns, int Vh, int p,
                                            T omega, int hits, int h) {
@@ -182,16 +185,16 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         mat3<T> uf, gf, ub, gb;
-        load_su3<true>(uf, u0_field(u0, Vh, p, i), h);
+        load_su3<STREAM>(uf, u0_field(u0, Vh, p, i), h);
         load_su3<false>(gf, gq, nb.fwd[i]);
-        load_su3<true>(ub, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
+        load_su3<STREAM>(ub, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
         load_su3<false>(gb, gq, nb.bwd[i]);
         m_mac_nd(W, uf, gf);  // u0_i(n) G(n+i)^dagger
         m_mac_dd(W, ub, gb);  // u0_i(n-i)^dagger G(n-i)^dagger
     }
     cplx<T> *gp = g_field(g, Vh, p);
     mat3<T> gn, w, upd, out;
-    load_su3<true>(gn, static_cast<const cplx<T> *>(gp), h);
+    load_su3<STREAM>(gn, static_cast<const cplx<T> *>(gp), h);
     m_mul(gn, W, w);
     la_or_update(w, hits, omega, upd);
     m_mul(upd, gn, out);
@@ -205,14 +208,14 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
 // which they stream through together.  (Remapping each XCD onto its own
 // contiguous range, so that z-neighbour G share one L2, measured 5% slower at
 // 48^3 x 96; runs of 4-32 workgroups per XCD measured the same as plain order.)
-template <typename T, bool POUT = false>
+template <typename T, bool POUT = false, bool STREAM = true>
 __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0,
                                                   cplx<T> *__restrict__ g, int ns, int Vh,
                                                   int p, T omega, int hits, int h0,
                                                   int count) {
     const int tid = (int)(blockIdx.x * kBlock + threadIdx.x);
     if (tid >= count) return;
-    sweep_site<T, POUT>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
+    sweep_site<T, POUT, STREAM>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
 }
 
 // ---------------------------------------------------------------------------

@@ -4,7 +4,10 @@ Between two host decisions the sweeps run group of timeslices by group of
 timeslices on several streams.  A site update reads only its own timeslice,
 so the gauge transformation, the sweep count and every residue must equal
 those of whole-lattice sweeps bit for bit, for any grouping (ragged last
-group, one group, one slice per group) and in both precisions.
+group, one group, one slice per group) and in both precisions.  The sweeps
+read u0 and G(n) through the cache when what the concurrent launches sweep
+fits in 192 MiB and with non-temporal loads otherwise; both load policies
+give the same bits.
 """
 import numpy as np
 import pytest
@@ -56,3 +59,25 @@ def test_blocked_stop_at_max_sweeps(ctx):
             assert rc == gfc.ENOTCONVERGED and rep.sweeps == 137
             out.append(job.download_gauge())
         assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("precision", [gfc.PREC_FP64, gfc.PREC_MIXED])
+def test_cached_and_streamed_loads_agree(ctx, precision):
+    # 48^3 x 16: one timeslice holds 40.5 MiB of u0 and G in FP64 (20.25 MiB
+    # in FP32), so whole-lattice sweeps and groups of 3 timeslices on four
+    # streams use non-temporal loads, groups of 1 and the default grouping
+    # cached ones
+    geom = gfc.Geometry(48, 16)
+    p = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=100000000,
+                   max_sweeps_to_fix=64, sweeps_to_reunitarization=10)
+    with gfc.Job(ctx, geom, precision) as job:
+        job.generate_smooth(8, 0.35)
+        job.reunitarize()
+        job.save_config()
+        o = dict(precision=precision, mixed_probe_every=30)
+        rc0, rep0, G0, _ = _run(job, p, gfc.Options(slice_group=-1, **o))
+        assert rc0 == gfc.ENOTCONVERGED and rep0.sweeps == 64
+        for grp in (1, 3, 0):
+            rc, rep, G, _ = _run(job, p, gfc.Options(slice_group=grp, **o))
+            assert rc == rc0 and rep.sweeps_fp32 == rep0.sweeps_fp32
+            assert np.array_equal(G, G0)
