@@ -685,12 +685,20 @@ struct SliceTracker {
             desc_fp32 = fp32;
         }
         const int n = n_items[p];
-        if (fp32)
-            hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3(n), dim3(kBlock), 0, s, d_job,
-                               d_items[p], n);
+        // cached loads while the active slices' data fit in the cache
+        const bool stream = (int64_t)n_active * slice_bytes(j, fp32) > kCacheBytes;
+        if (fp32 && stream)
+            hipLaunchKernelGGL((gfc::k_sweep_batch<float, true>), dim3(n), dim3(kBlock), 0, s,
+                               d_job, d_items[p], n);
+        else if (fp32)
+            hipLaunchKernelGGL((gfc::k_sweep_batch<float, false>), dim3(n), dim3(kBlock), 0, s,
+                               d_job, d_items[p], n);
+        else if (stream)
+            hipLaunchKernelGGL((gfc::k_sweep_batch<double, true>), dim3(n), dim3(kBlock), 0, s,
+                               d_job, d_items[p], n);
         else
-            hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3(n), dim3(kBlock), 0, s, d_job,
-                               d_items[p], n);
+            hipLaunchKernelGGL((gfc::k_sweep_batch<double, false>), dim3(n), dim3(kBlock), 0, s,
+                               d_job, d_items[p], n);
         LAUNCH_CHECK();
         return GFC_OK;
     }

@@ -643,8 +643,9 @@ struct BatchJob {
 };
 
 // One workgroup per item = (job index, first parity index, parity, unused).
-// (Looping a workgroup over several items measured 17% slower.)
-template <typename T>
+// (Looping a workgroup over several items measured 17% slower.)  STREAM as in
+// sweep_site.
+template <typename T, bool STREAM = true>
 __global__ __launch_bounds__(kBlock) void k_sweep_batch(const BatchJob *__restrict__ jobs,
                                                         const int4 *__restrict__ items,
                                                         int nitems) {
@@ -654,8 +655,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batch(const BatchJob *__restri
     const BatchJob J = jobs[it.x];
     const int h = it.y + (int)threadIdx.x;
     if (h < J.Vh)
-        sweep_site<T>(static_cast<const cplx<T> *>(J.u0), static_cast<cplx<T> *>(J.g), J.ns,
-                      J.Vh, it.z, (T)J.omega, J.hits, h);
+        sweep_site<T, false, STREAM>(static_cast<const cplx<T> *>(J.u0),
+                                     static_cast<cplx<T> *>(J.g), J.ns, J.Vh, it.z, (T)J.omega,
+                                     J.hits, h);
 }
 
 // item = (job index, block index within the lattice's [0, 2 Vh) range, fp32 G?, _)

@@ -52,6 +52,12 @@ enum State { QUEUED, INIT, RUN, DONE };
 
 constexpr int kLanes = gfc_ctx::kLanes;
 
+// A lane whose runners' u0 and G (192 B per site in FP32, 384 B in FP64) fit
+// in this share of the 256 MiB Infinity Cache sweeps with cached loads, so
+// that its data stays in the cache from one half-sweep to the next; larger
+// lanes stream with non-temporal loads.
+constexpr int64_t kLaneCacheBytes = 96ll << 20;
+
 // Size class of a lattice: its lane.
 int lane_of(const gfc_geom &g) {
     const int64_t V = gfc_geom_volume(&g);
@@ -323,8 +329,12 @@ int plan_and_run(gfc_server *s, Lane &L) {
         K = std::max(K, stop - r.sweep);
     }
     if (run.empty()) return GFC_OK;
-    // plan signature: (slot, job, fp32, steps, parity) of every runner
-    std::vector<int64_t> sig;
+    int64_t bytes = 0;
+    for (Req *r : run) bytes += r->job->V * (r->fp32 ? 192 : 384);
+    const bool cached = bytes <= kLaneCacheBytes;
+    // plan signature: load policy, then (slot, job, fp32, steps, parity) of
+    // every runner
+    std::vector<int64_t> sig{cached};
     for (size_t q = 0; q < run.size(); q++) {
         sig.push_back(run[q]->slot);
         sig.push_back((int64_t)(intptr_t)run[q]->job);
@@ -372,12 +382,19 @@ int plan_and_run(gfc_server *s, Lane &L) {
         for (size_t i = 0; i < seg.size(); i++) {
             const size_t off = seg[i].first, cnt = seg[i].second;
             if (cnt == 0) continue;
-            if (i % 2 == 0)
-                hipLaunchKernelGGL(gfc::k_sweep_batch<double>, dim3((unsigned)cnt), dim3(kBlock),
-                                   0, st, L.d_jobs, d_items + off, (int)cnt);
+            const dim3 grid((unsigned)cnt);
+            if (i % 2 == 0 && cached)
+                hipLaunchKernelGGL((gfc::k_sweep_batch<double, false>), grid, dim3(kBlock), 0,
+                                   st, L.d_jobs, d_items + off, (int)cnt);
+            else if (i % 2 == 0)
+                hipLaunchKernelGGL((gfc::k_sweep_batch<double, true>), grid, dim3(kBlock), 0,
+                                   st, L.d_jobs, d_items + off, (int)cnt);
+            else if (cached)
+                hipLaunchKernelGGL((gfc::k_sweep_batch<float, false>), grid, dim3(kBlock), 0, st,
+                                   L.d_jobs, d_items + off, (int)cnt);
             else
-                hipLaunchKernelGGL(gfc::k_sweep_batch<float>, dim3((unsigned)cnt), dim3(kBlock),
-                                   0, st, L.d_jobs, d_items + off, (int)cnt);
+                hipLaunchKernelGGL((gfc::k_sweep_batch<float, true>), grid, dim3(kBlock), 0, st,
+                                   L.d_jobs, d_items + off, (int)cnt);
         }
         hipError_t ce = hipStreamEndCapture(st, &g);
         if (ce != hipSuccess) return fail(GFC_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
