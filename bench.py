@@ -446,6 +446,9 @@ def main():
         "fp64_F": F64,
         "fp64_plaquette_drift": max(abs(plaq64[0] - plaq0[0]), abs(plaq64[1] - plaq0[1])),
         "sweep_kernel_ms": round(sweep_ms, 5),
+        # the fix itself runs cache-resident timeslice waves: its own rate in
+        # algorithmic bytes (2016 B per site update)
+        "fix_algorithmic_GBs": round(value * 2016 / 1e9, 1),
         **out_mixed,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
