@@ -156,9 +156,13 @@ typedef struct {
                                groups, so that kernel tails overlap.
                                Timeslices do not interact, so the result is
                                bit-identical to whole-lattice sweeps.
-                               0 (default): about 16 groups of at least 64K
-                               parity sites, or whole-lattice sweeps for
-                               small lattices; -1: whole-lattice sweeps.      */
+                               0 (default): groups sized so that a wave of
+                               four stays in 192 MiB of the Infinity Cache
+                               between half-sweeps (read with cached loads),
+                               whole-lattice sweeps for lattices that fit in
+                               it, single timeslices when a wave of them does
+                               not fit; -1: whole-lattice sweeps.  With
+                               retire_slices, one timeslice per group.        */
 } gfc_options;
 
 typedef struct {
