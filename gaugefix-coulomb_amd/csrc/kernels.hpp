@@ -31,8 +31,9 @@ constexpr int kPlanes = 6;  // stored complex elements per SU(3) matrix
 // that launches them (engine.hip, server.hip) carries its own code object.
 namespace {
 
-// Loads that stream (read once per sweep) carry the non-temporal hint so that
-// the L2 keeps the neighbour G matrices, which six sites read.
+// Loads that stream (read once per sweep of a lattice larger than the
+// Infinity Cache) carry the non-temporal hint so that the L2 keeps the
+// neighbour G matrices, which six sites read.
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef float fvec2 __attribute__((ext_vector_type(2)));
 
