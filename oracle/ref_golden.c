@@ -25,6 +25,17 @@
  *       return code, e2, theta, F, plaq spatial, plaq temporal, then G (V x 18)
  *       and U (4V x 18) when V <= 4096 else G at sites 0, 7, 13, ... (stride 7,
  *       up to 4096 sites).
+ *   trace CFG NS NT R KMAX OUT
+ *       the same trajectory as traj, observed after every half-sweep
+ *       k = 1..KMAX at the cost of one run (R even): U only evolves through
+ *       U (w and the site update never read G), so from the state after k
+ *       sweeps (k even) one gaugefixOverrelaxation call with
+ *       max_sweeps_to_fix = 1 on a copy gives the state after k + 1 and one
+ *       with max_sweeps_to_fix = 2 the state after k + 2 (both calls start
+ *       at sweep 0, even parity, and return before their own
+ *       reunitarization); the harness then reunitarizes U when k + 2 is a
+ *       multiple of R, as the continuous run does after that sweep.  Writes
+ *       K, e2, theta, F, plaq spatial, plaq temporal for each K.
  *   full CFG NS NT R EST TOL OUT
  *       reunitarize and fix to tolerance TOL with estimate EST; writes return
  *       code, e2, theta, F, plaq spatial, plaq temporal and G as in traj.
@@ -141,9 +152,12 @@ static void put_fields(Mtrx3x3 *U, Mtrx3x3 *G, int with_u) {
     }
 }
 
+/* omega 1.95 unless REF_OMEGA is set (fixtures of the degenerate branches of
+   the local update need omega = 1.5 or 1.0, where they stay exact). */
 static ORGaugeFixingParameters base_params(unsigned R) {
     ORGaugeFixingParameters p = initParametersORDefault();
-    p.omega_OR = 1.95f;
+    const char *om = getenv("REF_OMEGA");
+    p.omega_OR = om ? (float)atof(om) : 1.95f;
     p.hits = 2;
     p.generic_gf.sweeps_to_reunitarization = R;
     return p;
@@ -203,6 +217,32 @@ int main(int argc, char **argv) {
                 put(rc);
                 put_obs(U);
                 put_fields(U, G, 1);
+            }
+        } else if (!strcmp(mode, "trace")) {
+            Mtrx3x3 *U = load_or_die(argv[2]);
+            reunitarizeField(U, DIM * V);
+            const unsigned R = (unsigned)atoi(argv[5]);
+            const int kmax = atoi(argv[6]);
+            if (R % 2) {
+                fprintf(stderr, "trace needs an even R\n");
+                return 1;
+            }
+            Mtrx3x3 *Uc = allocate3x3Field(DIM * V), *G = allocate3x3Field(V);
+            ORGaugeFixingParameters p = base_params(R);
+            p.generic_gf.tolerance = 1e-300;
+            p.generic_gf.estimate_sweeps_to_gf_progress = 100000000;
+            for (int k = 0; k < kmax; k += 2) {
+                memcpy(Uc, U, sizeof(Mtrx3x3) * DIM * V);
+                p.generic_gf.max_sweeps_to_fix = 1;
+                gaugefixOverrelaxation(Uc, G, p);
+                put(k + 1);
+                put_obs(Uc);
+                if (k + 2 > kmax) break;
+                p.generic_gf.max_sweeps_to_fix = 2;
+                gaugefixOverrelaxation(U, G, p);
+                put(k + 2);
+                put_obs(U);
+                if ((unsigned)(k + 2) % R == 0) reunitarizeField(U, DIM * V);
             }
         } else if (!strcmp(mode, "full")) {
             Mtrx3x3 *U = load_or_die(argv[2]);

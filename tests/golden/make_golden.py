@@ -23,7 +23,15 @@ reference's code.  Outputs:
   lat_2x2.npz      2^3x2 (eps 0.3, seed 11) and 6^3x4 (eps 0.3, seed 12),
   lat_6x4.npz      the same records as lat_4x8.npz
   lat_32x64.npz    config 2 (32^3x64, eps 0.3, seed 3): observables after
-                   k in K_LARGE, G (sampled sites) after 200, run to tolerance
+                   every half-sweep k = 1..200 (harness mode `trace`, checked
+                   bit for bit against `traj` on the small lattices), G
+                   (sampled sites) after 200, run to tolerance
+  lat_degen_4x8.npz  4^3x8 smooth configuration whose spatial links on
+                   timeslices 0-2 are exact patterns that drive the local
+                   update into the reference's degenerate branches (see
+                   degenerate_config); trajectories for omega 1.5 and 1.0:
+                   observables after every half-sweep k = 1..200, G and U
+                   after k in KEEP_G
   io.npz           loadConfig results (SHA-256 of the loaded arrays and return
                    codes) for big-endian FP64, little-endian FP64 and
                    little-endian FP32 files, and for short / long files
@@ -48,7 +56,6 @@ import gfc  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 REF = os.path.join(ROOT, "oracle", "_ref")
 KEEP_G = [1, 2, 3, 10, 11, 200]
-K_LARGE = [1, 2, 3, 5, 10, 20, 50, 100, 200]
 
 
 def sha(a: np.ndarray) -> str:
@@ -98,8 +105,16 @@ def parse_full(data: np.ndarray, V: int):
     return rc, obs, G
 
 
+def trace(harness: str, path: str, ns: int, nt: int, kmax: int) -> np.ndarray:
+    """(k, e2, theta, F, plaq spatial, plaq temporal) after k = 1..kmax."""
+    d = run(harness, "trace", path, ns, nt, 10, kmax).reshape(-1, 6)
+    assert list(d[:, 0].astype(int)) == list(range(1, kmax + 1))
+    return d
+
+
 def lattice(tmp: str, name: str, ns: int, nt: int, eps: float, seed: int, ks, keep_g,
-            harness: str, full_runs=((1000, 1e-16), (0, 1e-16)), keep_u=(200,)):
+            harness: str, full_runs=((1000, 1e-16), (0, 1e-16)), keep_u=(200,),
+            trace_kmax: int = 0):
     geom = gfc.Geometry(ns, nt)
     V = geom.volume
     U = gfc.generate_smooth(geom, seed, eps)
@@ -125,6 +140,10 @@ def lattice(tmp: str, name: str, ns: int, nt: int, eps: float, seed: int, ks, ke
         if Uk is not None and k in keep_u:
             fx[f"U_{k}"] = Uk
     fx["G_sites"] = sampled_sites(V)
+    if trace_kmax:
+        d = trace(harness, path, ns, nt, trace_kmax)
+        fx["trace_k"] = d[:, 0].astype(int)
+        fx["trace_obs"] = d[:, 1:]
     for est, tol in full_runs:
         rc, obs, G = parse_full(run(harness, "full", path, ns, nt, 10, est, tol), V)
         fx[f"full_est{est}_rc"] = rc
@@ -133,6 +152,77 @@ def lattice(tmp: str, name: str, ns: int, nt: int, eps: float, seed: int, ks, ke
             fx[f"full_est{est}_G"] = G
     np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **fx)
     print(name, "done", {k: fx[k] for k in fx if k.startswith("full") and k.endswith("rc")})
+
+
+D_ZZ1 = np.diag([-1, -1, 1]).astype(np.complex128)
+D_1ZZ = np.diag([1, -1, -1]).astype(np.complex128)
+P_II = np.diag([1j, 1j, -1])
+
+
+def degenerate_config(ns: int = 4, nt: int = 8, seed: int = 31, eps: float = 0.3):
+    """A smooth configuration (gfc.generate_smooth) whose spatial links on
+    timeslices 0-2 are replaced by exact diagonal SU(3) patterns (temporal
+    links and timeslices 3.. stay random).  w(n) only couples links of one
+    timeslice, so each pattern keeps its slice in one branch of the local
+    update (src/gauge_fixing.c:537-596, 656-708) at every half-sweep:
+
+      t=0  U_i(n) = 1 for even n_i and diag(-1,-1,1) for odd n_i (i = Z, Y, X):
+           w = diag(0,0,6), det w = 0, so inverse3x3 gives the null matrix and
+           g_LA = 0 (:586-593); the OR update is Proj((1 - omega) 1), which is
+           diag(-1,-1,1) for omega 1.5 (it flips every spatial link of the
+           slice, and the pattern repeats on the other parity) and a failed
+           projection -> identity for omega 1.0 (:701-705);
+      t=1  U_Z = 1, U_Y = diag(-1,-1,1), U_X = diag(1,-1,-1) everywhere:
+           w = diag(2,-2,2); the R and T Cabibbo-Marinari vectors are null and
+           those subgroup steps are skipped (src/SU3_ops.c:829-839), S is the
+           identity, g = 1;
+      t=2  U_Z(n) = diag(i,i,-1) for even z and 1 for odd z, U_Y and U_X as on
+           t=0: w = diag(1 +- i, 1 +- i, 4), det w = +-8i, so inverse3x3's
+           double (real) return is 0 and no hit runs (:569); g = w w^-1 = 1.
+
+    With omega 1.5 or 1.0 every value on these slices stays exact (entries
+    0, +-1, +-i, 1/2), so reference and engine must agree on them bit for
+    bit."""
+    geom = gfc.Geometry(ns, nt)
+    U = gfc.generate_smooth(geom, seed, eps).reshape(nt, ns, ns, ns, 4, 3, 3)
+    one = np.eye(3, dtype=np.complex128)
+    idx = np.arange(ns)
+    for t in range(3):
+        for mu, axis in ((1, 0), (2, 1), (3, 2)):  # mu Z, Y, X -> array axes z, y, x
+            odd = (idx % 2 == 1)
+            shape = [1, 1, 1]
+            shape[axis] = ns
+            odd = odd.reshape(shape)
+            if t == 0 or (t == 2 and mu != 1):
+                pat = np.where(odd[..., None, None], D_ZZ1, one)
+            elif t == 1:
+                pat = np.broadcast_to({1: one, 2: D_ZZ1, 3: D_1ZZ}[mu], (1, 1, 1, 3, 3))
+            else:  # t == 2, mu == Z
+                pat = np.where(odd[..., None, None], one, P_II)
+            U[t, :, :, :, mu] = np.broadcast_to(pat, (ns, ns, ns, 3, 3))
+    return geom, np.ascontiguousarray(U.reshape(geom.volume, 4, 3, 3))
+
+
+def degenerate(tmp: str):
+    geom, U = degenerate_config()
+    ns, nt, V = geom.ns, geom.nt, geom.volume
+    path = cfg_file(tmp, geom, U, 8, False)
+    fx = {"ns": ns, "nt": nt, "seed": 31, "eps": 0.3, "config_sha256": sha(U), "R": 10,
+          "hits": 2}
+    for omega in ("1.5", "1.0"):
+        os.environ["REF_OMEGA"] = omega
+        tag = omega.replace(".", "")
+        t = parse_traj(run("ref_golden_le64", "traj", path, ns, nt, 10, *KEEP_G), V, KEEP_G)
+        fx[f"om{tag}_traj_k"] = np.array([r[0] for r in t])
+        fx[f"om{tag}_traj_rc"] = np.array([r[1] for r in t])
+        for k, _, _, G, Uk in t:
+            fx[f"om{tag}_G_{k}"] = G
+            fx[f"om{tag}_U_{k}"] = Uk
+        d = trace("ref_golden_le64", path, ns, nt, 200)
+        fx[f"om{tag}_trace_obs"] = d[:, 1:]
+    del os.environ["REF_OMEGA"]
+    np.savez_compressed(os.path.join(OUT, "lat_degen_4x8.npz"), **fx)
+    print("lat_degen_4x8 done")
 
 
 def prims():
@@ -249,9 +339,11 @@ def main():
             lattice(tmp, "lat_2x2", 2, 2, 0.3, 11, range(1, 201), KEEP_G, "ref_golden_le64")
         if not only or "6x4" in only:
             lattice(tmp, "lat_6x4", 6, 4, 0.3, 12, range(1, 201), KEEP_G, "ref_golden_le64")
+        if not only or "degen" in only:
+            degenerate(tmp)
         if (not only and not a.skip_large) or "32x64" in only:
-            lattice(tmp, "lat_32x64", 32, 64, 0.3, 3, K_LARGE, [200], "ref_golden_le64_mt",
-                    full_runs=((1000, 1e-16),), keep_u=())
+            lattice(tmp, "lat_32x64", 32, 64, 0.3, 3, [200], [200], "ref_golden_le64_mt",
+                    full_runs=((1000, 1e-16),), keep_u=(), trace_kmax=200)
 
 
 if __name__ == "__main__":

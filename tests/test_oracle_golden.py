@@ -152,3 +152,42 @@ def test_trajectory_8x16_config1():
         U = U0.copy()
         rc, _ = orc.gaugefix(U, G, g, orc.params(estimate=est, reunit_every=10))
         assert rc == f[f"full_est{est}_rc"]
+
+
+def _degenerate():
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_golden
+    f = golden("lat_degen_4x8.npz")
+    geom, U = make_golden.degenerate_config()
+    import hashlib
+    assert hashlib.sha256(U.tobytes()).hexdigest() == str(f["config_sha256"])
+    return f, geom, U
+
+
+@pytest.mark.parametrize("omega", ["1.5", "1.0"])
+def test_degenerate_branches(omega):
+    """det(w) = 0 (g_LA = 0, then diag(-1,-1,1) for omega 1.5 or a failed
+    projection -> identity for omega 1.0), null Cabibbo-Marinari vectors and
+    Re det(w) = 0 on timeslices 0-2 (tests/golden/make_golden.py
+    degenerate_config): the oracle reproduces the reference's G and U after k
+    sweeps, bit for bit on those slices, and its e2 after every sweep."""
+    f, geom, U0 = _degenerate()
+    tag = omega.replace(".", "")
+    g = orc.geom(geom.ns, geom.nt)
+    orc.reunitarize(U0)
+    S = geom.ns ** 3
+    for k in [1, 2, 3, 10, 11, 200]:
+        U = U0.copy()
+        G = np.zeros((geom.volume, 3, 3), np.complex128)
+        p = orc.params(tolerance=1e-300, estimate=100000000, max_sweeps=k, reunit_every=10,
+                       omega=float(omega))
+        rc, tr = orc.gaugefix(U, G, g, p, trace_len=k)
+        assert rc == -1
+        Gref, Uref = f[f"om{tag}_G_{k}"], f[f"om{tag}_U_{k}"]
+        assert np.array_equal(G[:3 * S], Gref[:3 * S])
+        assert np.max(np.abs(G - Gref)) < 1e-13
+        assert np.max(np.abs(U - Uref)) < 1e-13
+    ref = f[f"om{tag}_trace_obs"][:, 0]
+    assert np.max(np.abs(tr - ref) / ref) < 1e-13
