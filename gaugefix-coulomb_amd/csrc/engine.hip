@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,40 @@ using gfcx::nblocks;
 thread_local std::string gfcx::last_error;
 
 using namespace gfcx;
+
+// ---------------------------------------------------------------------------
+// Scratch of gfc_config_observables_noalloc: per-block partial sums of the
+// observable kernels (3 doubles per 256-site block, at most kNoallocBlocks
+// blocks) followed by the six reduced sums.  Each GPU holds its own copy for
+// the life of the process; its device address is looked up once per device
+// when a context is created and kept in noalloc_scratch[device].  Every
+// kernel writes every partial slot the following reduction reads, so nothing
+// has to be cleared between calls.
+namespace {
+constexpr int kNoallocBlocks = 43680;  // V <= 11 182 080 sites (48^3 x 96 fits)
+constexpr int kNoallocOut = 3 * kNoallocBlocks;
+constexpr int kNoallocDoubles = kNoallocOut + 8;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu;
+double *noalloc_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (which is `device`).
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(GFC_EDEVICE, "gfc_ctx_create: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_scratch[device] != nullptr) return GFC_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        return fail(GFC_EDEVICE, std::string("gfc_ctx_create: scratch address: ") +
+                                     hipGetErrorString(e));
+    noalloc_scratch[device] = static_cast<double *>(ptr);
+    return GFC_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -58,7 +93,7 @@ int gfc_ctx_create(int device, gfc_ctx **out) {
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->res_dev, 4 * sizeof(double));
-    if (e == hipSuccess) e = hipHostMalloc(&c->res_host, 4 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&c->res_host, 8 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&c->flag_dev, sizeof(int));
     c->lane[0] = c->stream;
     for (int k = 1; k < gfc_ctx::kLanes && e == hipSuccess; k++)
@@ -69,6 +104,11 @@ int gfc_ctx_create(int device, gfc_ctx **out) {
     if (e != hipSuccess) {
         gfc_ctx_destroy(c);
         return fail(GFC_EDEVICE, std::string("gfc_ctx_create: ") + hipGetErrorString(e));
+    }
+    const int rc = noalloc_setup(device);
+    if (rc != GFC_OK) {
+        gfc_ctx_destroy(c);
+        return rc;
     }
     *out = c;
     return GFC_OK;
@@ -1041,6 +1081,50 @@ extern "C" int gfc_reunitarize_field(gfc_ctx *ctx, double *field, int64_t n) {
     hipFree(d);
     if (rc == GFC_OK && flag) rc = fail(GFC_EPROJECT, "gfc_reunitarize_field: null row");
     return rc;
+}
+
+extern "C" int gfc_config_observables_noalloc(gfc_ctx *ctx, const gfc_geom *g,
+                                              const double *U_dev, double *e2, double *theta,
+                                              double *F, double *plaq_spatial,
+                                              double *plaq_temporal) {
+    if (ctx == nullptr || g == nullptr || U_dev == nullptr)
+        return fail(GFC_EINVAL, "gfc_config_observables_noalloc: NULL argument");
+    if (gfc_geom_check(g))
+        return fail(GFC_EGEOMETRY, "gfc_config_observables_noalloc: bad geometry");
+    const int64_t V = gfc_geom_volume(g);
+    const unsigned b = nblocks(V);
+    if (b > (unsigned)kNoallocBlocks)
+        return fail(GFC_EINVAL, "gfc_config_observables_noalloc: lattice larger than the "
+                                "scratch allows (at most 11182080 sites)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    double *ptr = noalloc_scratch[ctx->device];
+    if (ptr == nullptr)
+        return fail(GFC_EDEVICE, "gfc_config_observables_noalloc: scratch not set up");
+    hipStream_t s = ctx->stream;
+    const cplx<double> *cfg = reinterpret_cast<const cplx<double> *>(U_dev);
+    // the same kernels and block decomposition as gfc_job_observables and
+    // gfc_job_plaquette, with the partials and sums in the scratch
+    hipLaunchKernelGGL(gfc::k_obs_aos, dim3(b), dim3(kBlock), 0, s, cfg, g->ns, V, ptr);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(gfc::k_reduce_final, dim3(1), dim3(kBlock), 0, s, ptr, (int)b,
+                       ptr + kNoallocOut);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(gfc::k_plaquette_aos, dim3(b), dim3(kBlock), 0, s, cfg, g->ns, g->nt, V,
+                       ptr);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(gfc::k_reduce_final, dim3(1), dim3(kBlock), 0, s, ptr, (int)b,
+                       ptr + kNoallocOut + 3);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(ctx->res_host, ptr + kNoallocOut, 6 * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const double *r = ctx->res_host, dV = (double)V;
+    if (e2) *e2 = r[0] / dV;
+    if (theta) *theta = r[1] / (3.0 * dV);
+    if (F) *F = r[2] / (9.0 * dV);
+    if (plaq_spatial) *plaq_spatial = r[3] / (3.0 * dV);
+    if (plaq_temporal) *plaq_temporal = r[4] / (3.0 * dV);
+    return GFC_OK;
 }
 
 extern "C" int gfc_config_observables(gfc_ctx *ctx, const gfc_geom *g, const double *U,

@@ -50,7 +50,7 @@ struct gfc_ctx {
     double *part = nullptr;  // per-block partial sums
     size_t part_cap = 0;     // in blocks
     double *res_dev = nullptr;
-    double *res_host = nullptr;  // pinned, 4 doubles
+    double *res_host = nullptr;  // pinned, 8 doubles
     int *flag_dev = nullptr;
     // extra streams (default priority) over which independent timeslice
     // groups are spread, and the events that fork them from and join them

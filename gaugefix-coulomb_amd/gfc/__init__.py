@@ -162,6 +162,8 @@ def _load():
                                    C.POINTER(_Options), D, D, C.POINTER(_Report)]),
         "gfc_reunitarize_field": (I, [P, D, I64]),
         "gfc_config_observables": (I, [P, C.POINTER(_Geom), D, D, D, D, D, D]),
+        "gfc_config_observables_noalloc": (I, [P, C.POINTER(_Geom), C.c_void_p, D, D, D, D,
+                                               D]),
         "gfc_job_generate_smooth": (I, [P, C.c_uint64, C.c_double]),
         "gfc_job_load_config": (I, [P, C.c_char_p, I, I]),
         "gfc_job_write_gauge_transf": (I, [P, C.c_char_p]),
@@ -515,6 +517,16 @@ def config_observables(ctx: Context, geom: Geometry, U: np.ndarray):
     U = np.ascontiguousarray(U, dtype=np.complex128)
     _check(lib.gfc_config_observables(ctx._p, C.byref(g), _dptr(U),
                                       *[C.byref(v) for v in vals]))
+    return tuple(v.value for v in vals)
+
+
+def config_observables_noalloc(ctx: Context, geom: Geometry, U_dev: int):
+    """config_observables of a configuration already in device memory (U_dev:
+    a device address of 4V x 18 doubles); allocates nothing."""
+    g = geom._c()
+    vals = [C.c_double() for _ in range(5)]
+    _check(lib.gfc_config_observables_noalloc(ctx._p, C.byref(g), C.c_void_p(U_dev),
+                                              *[C.byref(v) for v in vals]))
     return tuple(v.value for v in vals)
 
 

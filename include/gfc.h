@@ -272,6 +272,19 @@ int gfc_config_observables(gfc_ctx *ctx, const gfc_geom *g, const double *U, dou
                            double *theta, double *F, double *plaq_spatial,
                            double *plaq_temporal);
 
+/* No counterpart in the reference.  gfc_config_observables on a configuration
+ * already in device memory (U_dev: 4*V*18 doubles, the AoS image above, on
+ * ctx's device), for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It allocates nothing: the per-block partial sums live in a fixed device
+ * array of the library (1 MiB per GPU, held until the process ends), so V is
+ * limited to 11182080 sites (GFC_EINVAL above that).  All calls on one GPU
+ * share that array: calls that overlap on one GPU (other threads or
+ * contexts) race.  Same values, bit for bit, as gfc_config_observables. */
+int gfc_config_observables_noalloc(gfc_ctx *ctx, const gfc_geom *g, const double *U_dev,
+                                   double *e2, double *theta, double *F,
+                                   double *plaq_spatial, double *plaq_temporal);
+
 /* ---- serving: continuous batching of many lattices (no counterpart in the
  * reference; each request runs exactly gfc_job_fix's algorithm) ------------- */
 typedef struct gfc_server gfc_server;
