@@ -83,3 +83,29 @@ def test_cached_and_streamed_loads_agree(ctx, precision, nt):
             rc, rep, G, _ = _run(job, p, gfc.Options(slice_group=grp, **o))
             assert rc == rc0 and rep.sweeps_fp32 == rep0.sweeps_fp32
             assert np.array_equal(G, G0)
+
+
+@pytest.mark.parametrize("ns,nt", [(10, 20), (14, 28)])
+@pytest.mark.parametrize("precision", [gfc.PREC_FP64, gfc.PREC_MIXED])
+def test_blocked_tiles_shared_across_timeslices(ctx, precision, ns, nt):
+    """ns = 2 mod 4: a timeslice (ns^3/2 parity sites) is not a whole number of
+    32-site tiles, so groups of timeslices on different streams write
+    different elements of the same G tiles and cache lines concurrently; the
+    result must still equal whole-lattice sweeps bit for bit."""
+    geom = gfc.Geometry(ns, nt)
+    assert (ns ** 3 // 2) % 32 != 0
+    p = gfc.Params(estimate_sweeps_to_gf_progress=400, sweeps_to_reunitarization=10)
+    with gfc.Job(ctx, geom, precision) as job:
+        job.generate_smooth(ns + 40, 0.3)
+        job.reunitarize()
+        job.save_config()
+        o = dict(precision=precision, mixed_probe_every=30)
+        rc0, rep0, G0, U0 = _run(job, p, gfc.Options(slice_group=-1, **o))
+        assert rc0 > 0
+        for grp in (1, 3, 0):
+            rc, rep, G, U = _run(job, p, gfc.Options(slice_group=grp, **o))
+            print(f"{ns}^3x{nt} precision {precision} group {grp}: {rc} sweeps")
+            assert rc == rc0 and rep.sweeps_fp32 == rep0.sweeps_fp32
+            assert rep.residue_final == rep0.residue_final
+            assert np.array_equal(G, G0)
+            assert np.array_equal(U, U0)

@@ -141,8 +141,8 @@ def test_drop_in_host_arrays(ctx):
 
 def test_config2_32x64(ctx, tmp_path):
     """Config 2: 32^3x64, eps 0.3, read from a little-endian FP64 SCIDAC file
-    (streamed into HBM) -- e2 after k sweeps, G after 200, plaquette
-    invariance and the sweep count to 1e-16 against the reference."""
+    (streamed into HBM) -- e2 after every half-sweep k = 1..200, G after 200,
+    plaquette invariance and the sweep count to 1e-16 against the reference."""
     f = golden("lat_32x64.npz")
     geom, U = _config(f)
     path = str(tmp_path / "cfg_32x64_le64.bin")
@@ -154,20 +154,23 @@ def test_config2_32x64(ctx, tmp_path):
     job.save_config()
     p0 = job.plaquette()
     assert np.allclose(p0, f["obs_reunit"][3:], atol=1e-14)
-    for i, k in enumerate(f["traj_k"]):
-        rc, rep = job.fix(_fixed_k(int(k)))
-        e2, th, F = job.observables()
-        ref = f["traj_obs"][i]
-        print(f"k={k} e2={e2:.6e} ref={ref[0]:.6e} rel={abs(e2 - ref[0]) / ref[0]:.2e}")
-        assert abs(e2 - ref[0]) / ref[0] < E2_REL
-        p1 = job.plaquette()
-        assert abs(p1[0] - p0[0]) < PLAQ_ABS and abs(p1[1] - p0[1]) < PLAQ_ABS
-        if k == 200:
-            G = job.download_gauge()
-            dG = np.max(np.abs(G[f["G_sites"]] - f["G_200"]))
-            print(f"G after 200: max|dG| = {dG:.3e}")
-            assert dG < G_ABS
-        job.restore_config()
+    rc, rep, tr = job.fix(_fixed_k(200), trace_len=200)
+    assert rc == gfc.ENOTCONVERGED and rep.sweeps == 200
+    ref = f["trace_obs"]
+    rel = np.abs(tr - ref[:, 0]) / ref[:, 0]
+    print(f"e2 after each of 200 half-sweeps: max rel diff {rel.max():.2e} "
+          f"(k = {int(np.argmax(rel)) + 1})")
+    assert rel.max() < E2_REL
+    e2, th, F = job.observables()
+    assert abs(th - ref[-1, 1]) / ref[-1, 1] < E2_REL and abs(F - ref[-1, 2]) < 1e-13
+    p1 = job.plaquette()
+    assert abs(p1[0] - p0[0]) < PLAQ_ABS and abs(p1[1] - p0[1]) < PLAQ_ABS
+    assert abs(p1[0] - ref[-1, 3]) < PLAQ_ABS and abs(p1[1] - ref[-1, 4]) < PLAQ_ABS
+    G = job.download_gauge()
+    dG = np.max(np.abs(G[f["G_sites"]] - f["G_200"]))
+    print(f"G after 200: max|dG| = {dG:.3e}")
+    assert dG < G_ABS
+    job.restore_config()
     rc, rep = job.fix(gfc.Params(sweeps_to_reunitarization=10))
     ref_rc = int(f["full_est1000_rc"])
     print(f"sweeps {rc} reference {ref_rc}")

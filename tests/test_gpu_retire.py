@@ -49,3 +49,31 @@ def test_retirement_needs_ns_multiple_of_8(ctx):
         with pytest.raises(gfc.GfcError) as e:
             job.fix(gfc.Params(), gfc.Options(retire_slices=1))
         assert e.value.code == gfc.EINVAL
+
+
+@pytest.mark.parametrize("precision", [gfc.PREC_FP64, gfc.PREC_MIXED])
+def test_retirement_against_reference(ctx, precision):
+    """Slice retirement on config 1 (8^3x16, the reference's fixture run to
+    1e-16 under its default schedule): the same sweep count, e2 <= 1e-16 and
+    the reference's gauge functional F to 1e-10 relative (retired slices stop
+    moving below the tolerance, so G is not compared)."""
+    import hashlib
+    from conftest import golden
+    f = golden("lat_8x16.npz")
+    geom = gfc.Geometry(int(f["ns"]), int(f["nt"]))
+    U = gfc.generate_smooth(geom, int(f["seed"]), float(f["eps"]))
+    assert hashlib.sha256(U.tobytes()).hexdigest() == str(f["config_sha256"])
+    p = gfc.Params(tolerance=1e-16, sweeps_to_reunitarization=10,
+                   estimate_sweeps_to_gf_progress=1000)
+    with gfc.Job(ctx, geom, precision) as job:
+        job.upload_config(U)
+        assert job.reunitarize() == 0
+        rc, rep = job.fix(p, gfc.Options(precision=precision, retire_slices=1,
+                                         mixed_probe_every=20))
+        e2, th, F = job.observables()
+    Fref = float(f["full_est1000_obs"][2])
+    print(f"precision {precision}: sweeps {rc} (reference {int(f['full_est1000_rc'])}), "
+          f"slice-sweeps {rep.slice_sweeps:.1f}, e2 {e2:.2e}, F rel {abs(F - Fref) / Fref:.2e}")
+    assert rc == int(f["full_est1000_rc"])
+    assert e2 <= 1e-16 and rep.slice_sweeps < rep.sweeps
+    assert abs(F - Fref) / abs(Fref) < 1e-10
