@@ -79,6 +79,7 @@ void gfc_options_default(gfc_options *o) {
     o->time_sweeps = 0;
     o->retire_slices = 0;
     o->slice_group = 0;
+    o->slice_lanes = 0;
 }
 
 int gfc_ctx_create(int device, gfc_ctx **out) {
@@ -428,11 +429,15 @@ int prepare_fix(gfc_job *j, bool mixed) {
     return GFC_OK;
 }
 
-int materialize(gfc_job *j) { return materialize(j, j->ctx->stream); }
+int materialize(gfc_job *j, bool project) { return materialize(j, j->ctx->stream, project); }
 
-int materialize(gfc_job *j, hipStream_t s) {
-    hipLaunchKernelGGL(gfc::k_materialize, dim3(nblocks(j->V)), dim3(kBlock), 0, s, j->cfg, j->g,
-                       j->geom.ns, j->geom.nt, j->Vh);
+int materialize(gfc_job *j, hipStream_t s, bool project) {
+    if (project)
+        hipLaunchKernelGGL(gfc::k_materialize<true>, dim3(nblocks(j->V)), dim3(kBlock), 0, s,
+                           j->cfg, j->g, j->geom.ns, j->geom.nt, j->Vh);
+    else
+        hipLaunchKernelGGL(gfc::k_materialize<false>, dim3(nblocks(j->V)), dim3(kBlock), 0, s,
+                           j->cfg, j->g, j->geom.ns, j->geom.nt, j->Vh);
     LAUNCH_CHECK();
     return GFC_OK;
 }
@@ -458,13 +463,20 @@ constexpr int64_t kCacheBytes = 192ll << 20;
 // 1 (FP64) and 2 (FP32) timeslices on four streams: cached loads make a
 // half-sweep 1% (FP64) and 9% (FP32) faster than the best streamed grouping;
 // on sets larger than the cache they are 4-5% slower.
-bool stream_loads(const gfc_job *j, bool fp32, int group) {
+bool stream_loads(const gfc_job *j, bool fp32, int group, int lanes) {
     const int64_t ts = slice_bytes(j, fp32);
     const int nt = j->geom.nt;
     if (group <= 0) return (int64_t)nt * ts > kCacheBytes;
     const int ngroups = (nt + group - 1) / group;
-    const int lanes = std::min(gfc_ctx::kLanes, ngroups);
+    lanes = std::min(lanes, ngroups);
     return (int64_t)lanes * group * ts > kCacheBytes;
+}
+
+// Streams over which the timeslice groups of precision fp32 run at once.
+int lane_count(const gfc_options &o, bool fp32) {
+    (void)fp32;
+    if (o.slice_lanes >= 1 && o.slice_lanes <= gfc_ctx::kLanes) return o.slice_lanes;
+    return gfc_ctx::kLanes;
 }
 
 template <typename T>
@@ -490,7 +502,7 @@ int launch_sweep(const cplx<T> *u0, cplx<T> *g, hipStream_t s, int ns, int Vh, i
 int half_sweep(gfc_job *j, SweepState &st, int parity, double omega, int hits) {
     hipStream_t s = j->ctx->stream;
     const int ns = j->geom.ns, Vh = j->Vh;
-    const bool stream = stream_loads(j, st.fp32, 0);
+    const bool stream = stream_loads(j, st.fp32, 0, 1);
     if (st.fp32)
         return launch_sweep(j->u0f, j->gf, s, ns, Vh, parity, (float)omega, hits, 0, Vh, false,
                             stream);
@@ -544,7 +556,7 @@ int group_size(const gfc_job *j, const gfc_options &o, bool fp32) {
     if (o.slice_group > 0) return o.slice_group < nt ? o.slice_group : nt;
     const int64_t ts = slice_bytes(j, fp32);
     if ((int64_t)nt * ts <= kCacheBytes) return 0;
-    const int64_t L = gfc_ctx::kLanes;
+    const int64_t L = lane_count(o, fp32);
     const int64_t nmax = kCacheBytes / (L * ts);
     if (nmax < 1) return 1;
     const int64_t waves = (nt + L * nmax - 1) / (L * nmax);
@@ -556,8 +568,8 @@ int group_size(const gfc_job *j, const gfc_options &o, bool fp32) {
 // s + 1 < to that is a multiple of R (folded into the sweeps around it).  Every site update reads only sites of
 // its own timeslice, so each group runs through the sweeps on its own and the
 // result equals that of whole-lattice sweeps bit for bit.
-int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group, double omega,
-                  int hits, unsigned R, const std::vector<char> *active = nullptr) {
+int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group, int max_lanes,
+                  double omega, int hits, unsigned R, const std::vector<char> *active = nullptr) {
     gfc_ctx *c = j->ctx;
     const int ns = j->geom.ns, nt = j->geom.nt, Vh = j->Vh;
     const int S = ns * ns * ns / 2;  // parity sites per timeslice
@@ -568,8 +580,8 @@ int sweep_blocked(gfc_job *j, const SweepState &st, int from, int to, int group,
         if (active == nullptr || (*active)[gi]) groups.push_back(gi);
     const int ng = (int)groups.size();
     if (ng == 0) return GFC_OK;
-    const int lanes = std::min(gfc_ctx::kLanes, ng);
-    const bool stream = stream_loads(j, st.fp32, group);
+    const int lanes = std::min(max_lanes, ng);
+    const bool stream = stream_loads(j, st.fp32, group, max_lanes);
     if (lanes > 1) {
         HIP_TRY(hipEventRecord(c->fork, c->stream));
         for (int l = 1; l < lanes; l++) HIP_TRY(hipStreamWaitEvent(c->lane[l], c->fork, 0));
@@ -845,8 +857,9 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
             if (st.fp32 && ho.switch_at > sweep) stop = std::min<int64_t>(stop, ho.switch_at);
             stop = std::min<int64_t>(stop, (int64_t)p->max_sweeps_to_fix);
             if (stop <= sweep) stop = sweep + 1;
-            RC_TRY(sweep_blocked(j, st, sweep, (int)stop, group, omega, hits,
-                                 p->sweeps_to_reunitarization, sl.on ? &sl.active : nullptr));
+            RC_TRY(sweep_blocked(j, st, sweep, (int)stop, group, lane_count(o, st.fp32), omega,
+                                 hits, p->sweeps_to_reunitarization,
+                                 sl.on ? &sl.active : nullptr));
             if (sl.on) sl.slice_sweeps += (double)(stop - sweep) * sl.n_active / sl.nt;
             if (in32) r.sweeps_fp32 += (int)stop - sweep;
             sweep = (int)stop;
@@ -911,8 +924,10 @@ int run_fix(gfc_job *j, const gfc_params *p, const gfc_options *o_in, gfc_report
     // left the loop in the FP32 phase (tolerance reached there or max sweeps):
     // hand the state over so that G and U are FP64
     if (st.fp32) RC_TRY(promote_g(j, st));
+    // converged: the final reunitarization of G and U (src/gauge_fixing.c:822-823);
+    // the max-sweeps and stall exits return before it
     if (!done) RC_TRY(reunit_g(j, false, 0, 2));
-    RC_TRY(materialize(j));
+    RC_TRY(materialize(j, !done));
     HIP_TRY(hipStreamSynchronize(j->ctx->stream));
     r.seconds_sweeps = timer.seconds();
     r.sweeps = sweep;

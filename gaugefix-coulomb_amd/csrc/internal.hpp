@@ -86,8 +86,8 @@ int set_identity(gfc_job *j);
 int set_identity(gfc_job *j, hipStream_t s);
 int build_u0(gfc_job *j);
 int build_u0(gfc_job *j, hipStream_t s);
-int materialize(gfc_job *j);
-int materialize(gfc_job *j, hipStream_t s);
+int materialize(gfc_job *j, bool project);
+int materialize(gfc_job *j, hipStream_t s, bool project);
 int reunit_g(gfc_job *j, bool fp32, int p0, int np);
 int reunit_g(gfc_job *j, hipStream_t s, bool fp32, int p0, int np);
 int promote_g_only(gfc_job *j);

@@ -127,6 +127,19 @@ __device__ __forceinline__ void store_su3(const mat3<T> &m, cplx<T> *__restrict_
     }
 }
 
+// The reunitarization of G (reunitarizeField, src/fields.c:144-187) always
+// projects in FP64: an FP32 G (the FP32 phase of the mixed mode) is promoted,
+// projected and rounded back.  The projection reads rows 0 and 1 only.
+__device__ __forceinline__ void project_g(mat3<double> &m) { m_project_su3(m); }
+__device__ __forceinline__ void project_g(mat3<float> &m) {
+    mat3<double> d;
+#pragma unroll
+    for (int k = 0; k < kPlanes; k++) d.e[k] = c_make<double>(m.e[k].re, m.e[k].im);
+    m_project_su3(d);
+#pragma unroll
+    for (int k = 0; k < kPlanes; k++) m.e[k] = c_make<float>((float)d.e[k].re, (float)d.e[k].im);
+}
+
 // Site geometry of parity index h on parity p.
 struct Site {
     int x, y, z, t, xh, row;
@@ -168,9 +181,9 @@ __device__ __forceinline__ Nbr neighbours(const Site &s, int h, int ns) {
 // The site update of parity index h on parity p: reads 6 u0 links, 6
 // neighbour G and G(n); writes G(n).  Shared by the single-lattice and the
 // batched (serving) kernels, so a site computes the same bits in both.
-// POUT projects the new G(n) onto SU(3) before storing it: the same bits as a
-// reunitarization of this parity's G right after the sweep, since the
-// projection reads rows 0 and 1 only.  STREAM reads u0 and G(n) with
+// POUT projects the new G(n) onto SU(3) (in FP64, project_g) before storing
+// it: the same bits as a reunitarization of this parity's G right after the
+// sweep, since the projection reads rows 0 and 1 only.  STREAM reads u0 and G(n) with
 // non-temporal loads (data read once per sweep); without it they are cached,
 // for sweeps whose data stays in the Infinity Cache from one half-sweep to the
 // next.
@@ -199,7 +212,7 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
     m_mul(gn, W, w);
     la_or_update(w, hits, omega, upd);
     m_mul(upd, gn, out);
-    if (POUT) m_project_su3(out);
+    if (POUT) project_g(out);
     store_su3<true>(out, gp, h);
 }
 
@@ -250,8 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_g_identity(cplx<T> *__restrict__ g, 
     store_su3<false>(m, g_field(g, Vh, p), h);
 }
 
-// Projects G onto SU(3) on parities [p0, p0 + np) (reunitarizeField on G,
-// src/fields.c:144-187).  The projection reads rows 0 and 1 only.
+// Projects G onto SU(3) at parity index h of parity p.
 template <typename T>
 __device__ __forceinline__ void reunit_site(cplx<T> *__restrict__ g, int Vh, int p, int h) {
     cplx<T> *base = g_field(g, Vh, p);
@@ -259,7 +271,7 @@ __device__ __forceinline__ void reunit_site(cplx<T> *__restrict__ g, int Vh, int
     mat3<T> m;
 #pragma unroll
     for (int k = 0; k < kPlanes; k++) m.e[k] = *elem_ptr(base, off, k);
-    m_project_su3(m);
+    project_g(m);
     store_su3<false>(m, base, h);
 }
 
@@ -320,7 +332,10 @@ __global__ __launch_bounds__(kBlock) void k_build_u0(const cplx<double> *__restr
 }
 
 // cfg <- G(n) cfg G(n+mu)^dagger for all four directions
-// (applyGaugeTransformationU, src/fields.c:375-435).
+// (applyGaugeTransformationU, src/fields.c:375-435); with PROJECT every link
+// is then projected onto SU(3), the reunitarization of U that ends a
+// converged gaugefixOverrelaxation (src/gauge_fixing.c:822).
+template <bool PROJECT>
 __global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict__ cfg,
                                                         const cplx<double> *__restrict__ g,
                                                         int ns, int nt, int Vh) {
@@ -349,6 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict
         for (int k = 0; k < 9; k++) u.e[k] = ptr[k];
         m_mul(gn, u, tmp);
         m_mul_nd(tmp, gm, out);
+        if (PROJECT) m_project_su3(out);
 #pragma unroll
         for (int k = 0; k < 9; k++) ptr[k] = out.e[k];
     }

@@ -1,9 +1,12 @@
 // server.hip -- continuous-batching gauge-fix server (gfc_server_* in gfc.h).
 //
-// Requests (lattice size, seed, eps, parameters, precision) queue on the host.
-// A request is admitted when a job slot and its HBM footprint are free: its
-// configuration is synthesised in HBM, reunitarized, and its fixing state is
-// the one gfc_job_fix keeps (sweep count, check schedule, FP32/FP64 phase).
+// Requests (lattice size, parameters, precision, and either a configuration
+// supplied by the caller or a seed and eps to synthesise one) queue on the
+// host.  A request is admitted when a job slot and its HBM footprint are free:
+// its configuration is copied (or synthesised) into HBM and reunitarized --
+// an input with a null row fails with GFC_EPROJECT, as the CLI does -- and its
+// fixing state is the one gfc_job_fix keeps (sweep count, check schedule,
+// FP32/FP64 phase).
 //
 // Lattices are served in four lanes by size (fewer than 2^18, 2^20, 2^22
 // sites, and larger), one stream each, so that a small lattice is never
@@ -70,6 +73,10 @@ struct Req {
     gfc_geom geom{0, 0};
     uint64_t seed = 0;
     double eps = 0;
+    // a caller-supplied configuration (4V x 18 reals of in_prec bytes), kept
+    // until its copy to the device has completed; empty: synthesise from seed
+    std::vector<char> host_cfg;
+    int in_prec = 0;
     gfc_params p{};
     gfc_options o{};
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -112,6 +119,9 @@ struct Lane {
     double *d_part = nullptr;
     size_t part_cap = 0;
     double *d_out = nullptr, *h_out = nullptr;
+    // reunitarization failure flag of each slot's input (set by k_reunit_aos at
+    // admission, read back with the slot's first residue)
+    int *d_flags = nullptr, *h_flags = nullptr;
     std::vector<int64_t> plan_sig;
     hipEvent_t chunk_ev[2] = {nullptr, nullptr};
     int64_t chunks = 0;
@@ -225,10 +235,28 @@ int admit_one(gfc_server *s, Lane &L, Req &r, bool &admitted) {
     int slot = 0;
     while (busy[slot]) slot++;
     r.slot = slot;
-    // input: synthesised in HBM, reunitarized like the CLI does after loading
-    RC_TRY(gfcx::generate_smooth_device(job, L.st, r.seed, r.eps));
+    // input: the caller's configuration or one synthesised in HBM,
+    // reunitarized like the CLI does after loading (gauge_fix_coulomb.c:126)
+    if (!r.host_cfg.empty()) {
+        const int64_t nreal = job->V * 4 * 18;
+        if (r.in_prec == 8) {
+            HIP_TRY(hipMemcpyAsync(job->cfg, r.host_cfg.data(), nreal * sizeof(double),
+                                   hipMemcpyHostToDevice, L.st));
+        } else {
+            // FP32 staging in the u0 planes (rebuilt below), as gfc_job_upload_config
+            float *stage = reinterpret_cast<float *>(job->u0);
+            HIP_TRY(hipMemcpyAsync(stage, r.host_cfg.data(), nreal * sizeof(float),
+                                   hipMemcpyHostToDevice, L.st));
+            hipLaunchKernelGGL(gfc::k_f32_to_f64, dim3(nblocks(nreal)), dim3(kBlock), 0, L.st,
+                               stage, reinterpret_cast<double *>(job->cfg), nreal);
+            LAUNCH_CHECK();
+        }
+    } else {
+        RC_TRY(gfcx::generate_smooth_device(job, L.st, r.seed, r.eps));
+    }
+    HIP_TRY(hipMemsetAsync(L.d_flags + slot, 0, sizeof(int), L.st));
     hipLaunchKernelGGL(gfc::k_reunit_aos, dim3(nblocks(job->V * 4)), dim3(kBlock), 0, L.st,
-                       job->cfg, job->V * 4, s->ctx->flag_dev);
+                       job->cfg, job->V * 4, L.d_flags + slot);
     LAUNCH_CHECK();
     RC_TRY(gfcx::build_u0(job, L.st));
     RC_TRY(gfcx::set_identity(job, L.st));
@@ -272,6 +300,8 @@ int evaluate(gfc_server *s, Lane &L, std::vector<Req *> &which) {
     LAUNCH_CHECK();
     HIP_TRY(hipMemcpyAsync(L.h_out, L.d_out, 3 * which.size() * sizeof(double),
                            hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(L.h_flags, L.d_flags, s->max_jobs * sizeof(int),
+                           hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     for (size_t q = 0; q < which.size(); q++) {
         Req *r = which[q];
@@ -298,10 +328,17 @@ int promote(Lane &L, Req &r) {
 int finish(Lane &L, Req &r, int rc) {
     if (r.fp32) RC_TRY(promote(L, r));
     if (rc >= 0) RC_TRY(gfcx::reunit_g(r.job, L.st, false, 0, 2));
-    RC_TRY(gfcx::materialize(r.job, L.st));
+    RC_TRY(gfcx::materialize(r.job, L.st, rc >= 0));
     r.rc = rc >= 0 ? r.sweep : rc;
     r.state = DONE;
     return GFC_OK;
+}
+
+// A request that cannot be fixed (its input could not be reunitarized): no
+// sweep, no result, the error code as its status.
+void fail_request(Req &r, int code) {
+    r.rc = code;
+    r.state = DONE;
 }
 
 int next_multiple(int s, int m) { return (s / m + 1) * m; }
@@ -500,6 +537,13 @@ int poll_lane(gfc_server *s, int li) {
         Req &r = *a;
         const int which = r.p.gfix_proxy == GFC_PROXY_THETA ? 1 : 0;
         if (r.state == INIT) {
+            // the input's copy to the device has completed (evaluate synchronised
+            // the lane)
+            std::vector<char>().swap(r.host_cfg);
+            if (L.h_flags[r.slot]) {
+                fail_request(r, GFC_EPROJECT);
+                continue;
+            }
             r.res = r.res0 = r.obs[which];
             r.F = r.obs[2];
             r.due = gfc_next_check(0, r.res, &r.p);
@@ -603,6 +647,8 @@ void free_lane(Lane &L) {
     hipFree(L.d_sel);
     hipFree(L.d_part);
     hipFree(L.d_out);
+    hipFree(L.d_flags);
+    hipHostFree(L.h_flags);
     hipHostFree(L.h_jobs);
     hipHostFree(L.h_oitems);
     hipHostFree(L.h_sel);
@@ -637,7 +683,11 @@ int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_r
         L.d_jobs = dev_alloc<BatchJob>(max_jobs);
         L.d_sel = dev_alloc<int>(max_jobs);
         L.d_out = dev_alloc<double>(3 * (size_t)max_jobs);
-        if (!L.d_jobs || !L.d_sel || !L.d_out) return fail(GFC_ENOMEM, "gfc_server_create");
+        L.d_flags = dev_alloc<int>(max_jobs);
+        if (!L.d_jobs || !L.d_sel || !L.d_out || !L.d_flags)
+            return fail(GFC_ENOMEM, "gfc_server_create");
+        HIP_TRY(hipHostMalloc(&L.h_flags, sizeof(int) * max_jobs, hipHostMallocDefault));
+        std::memset(L.h_flags, 0, sizeof(int) * max_jobs);
         HIP_TRY(hipHostMalloc(&L.h_jobs, sizeof(BatchJob) * max_jobs, hipHostMallocDefault));
         HIP_TRY(hipHostMalloc(&L.h_sel, sizeof(int) * max_jobs, hipHostMallocDefault));
         HIP_TRY(hipHostMalloc(&L.h_out, sizeof(double) * 3 * max_jobs, hipHostMallocDefault));
@@ -664,18 +714,20 @@ void gfc_server_destroy(gfc_server *s) {
 
 double gfc_server_clock(gfc_server *s) { return s ? s->clock() : 0.0; }
 
-int64_t gfc_server_submit(gfc_server *s, const gfc_geom *g, uint64_t seed, double eps,
-                          const gfc_params *p, const gfc_options *o, uint64_t tag) {
+}  // extern "C"
+
+namespace {
+
+// The checks and queueing shared by both submit calls; fill sets the input.
+template <typename Fill>
+int64_t submit_req(gfc_server *s, const gfc_geom *g, const gfc_params *p, const gfc_options *o,
+                   uint64_t tag, const char *who, Fill fill) {
     if (s == nullptr || g == nullptr || p == nullptr)
-        return fail(GFC_EINVAL, "gfc_server_submit: NULL argument");
-    if (gfc_geom_check(g)) return fail(GFC_EGEOMETRY, "gfc_server_submit: bad geometry");
-    if (!gfc_params_valid(p)) return fail(GFC_EPARAMS, "gfc_server_submit: invalid parameters");
+        return fail(GFC_EINVAL, std::string(who) + ": NULL argument");
+    if (gfc_geom_check(g)) return fail(GFC_EGEOMETRY, std::string(who) + ": bad geometry");
+    if (!gfc_params_valid(p)) return fail(GFC_EPARAMS, std::string(who) + ": invalid parameters");
     auto r = std::make_unique<Req>();
-    r->id = s->next_id++;
-    r->tag = tag;
     r->geom = *g;
-    r->seed = seed;
-    r->eps = eps;
     r->p = *p;
     if (o)
         r->o = *o;
@@ -686,12 +738,46 @@ int64_t gfc_server_submit(gfc_server *s, const gfc_geom *g, uint64_t seed, doubl
     r->lane = lane_of(r->geom);
     r->footprint = footprint_of(*r);
     if (r->footprint > s->budget)
-        return fail(GFC_ENOMEM, "gfc_server_submit: lattice larger than the memory budget");
+        return fail(GFC_ENOMEM, std::string(who) + ": lattice larger than the memory budget");
+    RC_TRY(fill(*r));
+    r->id = s->next_id++;
+    r->tag = tag;
     r->t_submit = s->clock();
     const int64_t id = r->id;
     s->queue.push_back(std::move(r));
     s->stats.submitted++;
     return id;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t gfc_server_submit(gfc_server *s, const gfc_geom *g, uint64_t seed, double eps,
+                          const gfc_params *p, const gfc_options *o, uint64_t tag) {
+    return submit_req(s, g, p, o, tag, "gfc_server_submit", [&](Req &r) {
+        r.seed = seed;
+        r.eps = eps;
+        return GFC_OK;
+    });
+}
+
+int64_t gfc_server_submit_config(gfc_server *s, const gfc_geom *g, const void *U,
+                                 int in_precision, const gfc_params *p, const gfc_options *o,
+                                 uint64_t tag) {
+    return submit_req(s, g, p, o, tag, "gfc_server_submit_config", [&](Req &r) {
+        if (U == nullptr) return fail(GFC_EINVAL, "gfc_server_submit_config: NULL U");
+        if (in_precision != 4 && in_precision != 8)
+            return fail(GFC_EINVAL, "gfc_server_submit_config: precision must be 4 or 8");
+        const size_t bytes = (size_t)gfc_geom_volume(g) * 4 * 18 * (size_t)in_precision;
+        try {
+            r.host_cfg.assign(static_cast<const char *>(U), static_cast<const char *>(U) + bytes);
+        } catch (const std::bad_alloc &) {
+            return fail(GFC_ENOMEM, "gfc_server_submit_config: host copy of the input");
+        }
+        r.in_prec = in_precision;
+        return GFC_OK;
+    });
 }
 
 int gfc_server_poll(gfc_server *s, gfc_completion *out, int max_out) {

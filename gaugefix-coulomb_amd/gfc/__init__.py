@@ -71,6 +71,7 @@ class _Options(C.Structure):
         ("time_sweeps", C.c_int),
         ("retire_slices", C.c_int),
         ("slice_group", C.c_int),
+        ("slice_lanes", C.c_int),
     ]
 
 
@@ -173,6 +174,9 @@ def _load():
         "gfc_server_clock": (C.c_double, [P]),
         "gfc_server_submit": (I64, [P, C.POINTER(_Geom), C.c_uint64, C.c_double,
                                     C.POINTER(_Params), C.POINTER(_Options), C.c_uint64]),
+        "gfc_server_submit_config": (I64, [P, C.POINTER(_Geom), C.c_void_p, I,
+                                           C.POINTER(_Params), C.POINTER(_Options),
+                                           C.c_uint64]),
         "gfc_server_poll": (I, [P, C.POINTER(_Completion), I]),
         "gfc_server_pending": (I, [P]),
         "gfc_server_get_stats": (I, [P, C.POINTER(_ServerStats)]),
@@ -268,11 +272,12 @@ class Options:
     time_sweeps: int = 0
     retire_slices: int = 0
     slice_group: int = 0
+    slice_lanes: int = 0
 
     def _c(self) -> _Options:
         return _Options(self.precision, self.mixed_switch, self.mixed_probe_every,
                         self.mixed_defer, self.verbose, self.time_sweeps, self.retire_slices,
-                        self.slice_group)
+                        self.slice_group, self.slice_lanes)
 
 
 @dataclass
@@ -586,6 +591,23 @@ class Server:
         o = (options or Options())._c()
         return _check(lib.gfc_server_submit(self._p, C.byref(g), seed, eps, C.byref(p),
                                             C.byref(o), tag))
+
+    def submit_config(self, geom: Geometry, U: np.ndarray, params: Params,
+                      options: Options | None = None, tag: int = 0) -> int:
+        """Queues the caller's configuration U (complex128 or complex64, shape
+        (V, 4, 3, 3)); the server keeps its own copy."""
+        if U.dtype == np.complex64:
+            prec = 4
+        else:
+            U = np.asarray(U, dtype=np.complex128)
+            prec = 8
+        assert U.shape == (geom.volume, 4, 3, 3)
+        U = np.ascontiguousarray(U)
+        g, p = geom._c(), params._c()
+        o = (options or Options())._c()
+        return _check(lib.gfc_server_submit_config(self._p, C.byref(g),
+                                                   U.ctypes.data_as(C.c_void_p), prec,
+                                                   C.byref(p), C.byref(o), tag))
 
     def poll(self) -> list:
         n = _check(lib.gfc_server_poll(self._p, self._buf, len(self._buf)))

@@ -163,6 +163,10 @@ typedef struct {
                                it, single timeslices when a wave of them does
                                not fit; -1: whole-lattice sweeps.  With
                                retire_slices, one timeslice per group.        */
+    int slice_lanes;        /* streams over which the timeslice groups run
+                               concurrently (a wave holds one group per
+                               stream): 1-4, or 0 (default) for the measured
+                               best per precision                             */
 } gfc_options;
 
 typedef struct {
@@ -292,7 +296,8 @@ typedef struct gfc_server gfc_server;
 typedef struct {
     int64_t id;
     uint64_t tag;
-    int status;             /* sweep count, or GFC_ENOTCONVERGED               */
+    int status;             /* sweep count, GFC_ENOTCONVERGED, or GFC_EPROJECT
+                               (input not reunitarizable)                       */
     int sweeps;             /* half-sweeps performed                           */
     int sweeps_fp32;        /* of which in FP32                                 */
     int ns, nt;
@@ -327,6 +332,15 @@ double gfc_server_clock(gfc_server *server);
  * Returns the request id (> 0) or an error code. */
 int64_t gfc_server_submit(gfc_server *server, const gfc_geom *g, uint64_t seed, double eps,
                           const gfc_params *p, const gfc_options *o, uint64_t tag);
+/* Queues a request for the caller's configuration U (4*V matrices of doubles,
+ * in_precision 8, or floats, 4, in the reference's image), copied at once so
+ * that the caller may free it.  At admission it is copied to HBM and
+ * reunitarized like the CLI does after loadConfig (gauge_fix_coulomb.c:126);
+ * a link with a null row ends the request with status GFC_EPROJECT and no
+ * sweep.  Returns the request id (> 0) or an error code. */
+int64_t gfc_server_submit_config(gfc_server *server, const gfc_geom *g, const void *U,
+                                 int in_precision, const gfc_params *p, const gfc_options *o,
+                                 uint64_t tag);
 /* One scheduling round: admit what fits, evaluate the residues that are due,
  * apply the per-lattice decisions, retire finished lattices and launch one
  * chunk of packed half-sweeps.  Writes up to max_out completions; returns
