@@ -72,12 +72,33 @@ class _CStdoutToStderr:
         return False
 
 
+def _host_has_avx512() -> bool:
+    """The ISA extensions -march=skylake-avx512 code uses (F, CD, BW, DQ, VL)."""
+    try:
+        flags = set()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                flags = set(line.split(":", 1)[1].split())
+                break
+        return {"avx512f", "avx512cd", "avx512bw", "avx512dq", "avx512vl"} <= flags
+    except OSError:
+        return False
+
+
 def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
     """Times the reference's gaugefixOverrelaxation (oracle/_ref build) on n
-    half-sweeps of the same lattice; falls back to the C oracle port."""
+    half-sweeps of the same lattice; falls back to the C oracle port.  The
+    reference is the build with its own Makefile flags (-O4
+    -march=skylake-avx512, Makefile:8) where the host has AVX-512, else the
+    portable -O2 build."""
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     vh = ns ** 3 * nt // 2
-    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_le64_mt.so")
+    perf_so = os.path.join(ROOT, "oracle", "_ref", "libref_le64_perf.so")
+    if os.path.exists(perf_so) and _host_has_avx512():
+        ref_so, build = perf_so, "-std=c99 -O4 -march=skylake-avx512 -mtune=skylake-avx512 -fopenmp"
+    else:
+        ref_so, build = (os.path.join(ROOT, "oracle", "_ref", "libref_le64_mt.so"),
+                         "-std=c99 -O2 -fopenmp (host without AVX-512)")
     Uw = U.copy()
     G = np.zeros((U.shape[0], 3, 3), np.complex128)
     dp = lambda a: a.ctypes.data_as(C.c_void_p)
@@ -118,6 +139,7 @@ def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
             orc.gaugefix(Uw, G, g, p)
             return time.perf_counter() - t
         kind = "port"
+        build = "oracle/gfc_oracle.c, -std=c99 -O2 -ffp-contract=off -fopenmp"
     with _CStdoutToStderr():
         t1 = run(1)  # setup (G <- 1, initial e2) + one half-sweep
         per = max(t1 * 0.5, 1e-3)
@@ -126,6 +148,7 @@ def cpu_baseline(U: np.ndarray, ns: int, nt: int, seconds: float):
     per_sweep = max(tn - t1, 1e-9) / n
     return {"value": vh / per_sweep, "unit": "site-updates/s", "cores": threads,
             "kind": kind,
+            "build": build,
             "sample": f"{n} half-sweeps of {ns}^3x{nt} (FP64, omega 1.95, hits 2) after one "
                       f"warm-up call; time of the (1+{n})-sweep call minus the 1-sweep call"}
 
@@ -143,6 +166,22 @@ def profile_kernels(ctx, job, n: int):
     rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0,
                                          mixed_probe_every=10 ** 6, slice_group=-1))
     log(f"profile: {rep.sweeps_fp32} FP32 half-sweeps")
+    sec = ctx.copy_probe(2 ** 30, reps=3)
+    log(f"profile: copied 1 GiB x 4 with k_copy16 (calibration kernel), "
+        f"{2 * 2 ** 30 / sec / 1e9:.1f} GB/s read+write")
+
+
+def profile_fix(ctx, job, n: int):
+    """n FP64 half-sweeps run as the timed fix runs them (default grouping:
+    cache-resident timeslice waves on several streams, reunitarization folded
+    in every 10 sweeps), then 1 GiB copies: counters for the dominant kernel of
+    the timed region and their calibration."""
+    import gfc
+    fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=100000000,
+                       max_sweeps_to_fix=n, sweeps_to_reunitarization=10)
+    job.restore_config()
+    rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_FP64))
+    log(f"profile: {rep.sweeps} FP64 half-sweeps in cache-resident timeslice waves")
     sec = ctx.copy_probe(2 ** 30, reps=3)
     log(f"profile: copied 1 GiB x 4 with k_copy16 (calibration kernel), "
         f"{2 * 2 ** 30 / sec / 1e9:.1f} GB/s read+write")
@@ -265,6 +304,10 @@ def main():
     ap.add_argument("--profile-kernels", type=int, default=0, metavar="N",
                     help="only run N FP64 and N FP32 half-sweeps plus a 1 GiB device copy "
                          "(calibration for rocprofv3 PMC counters), then exit")
+    ap.add_argument("--profile-fix", type=int, default=0, metavar="N",
+                    help="only run N FP64 half-sweeps the way the timed fix runs them, plus "
+                         "a 1 GiB device copy (rocprofv3 counters of the timed kernels), "
+                         "on an input synthesised in HBM, then exit")
     ap.add_argument("--device-input", action="store_true",
                     help="with --profile-kernels: synthesise the input in HBM")
     a = ap.parse_args()
@@ -278,6 +321,15 @@ def main():
     geom = gfc.Geometry(a.ns, a.nt)
     V = geom.volume
     vh = V // 2
+    if a.profile_fix:
+        ctx = gfc.Context(0)
+        with gfc.Job(ctx, geom, gfc.PREC_FP64) as job:
+            job.generate_smooth(a.seed, a.eps)
+            job.reunitarize()
+            job.save_config()
+            profile_fix(ctx, job, a.profile_fix)
+        ctx.close()
+        return
     if a.profile_kernels and a.device_input:
         ctx = gfc.Context(0)
         with gfc.Job(ctx, geom, gfc.PREC_MIXED) as job:
@@ -404,9 +456,14 @@ def main():
         from gfc import serve
         reqs = serve.stream_requests(n=a.serve, rate=a.serve_rate)
         log(f"serving {a.serve} requests, Poisson rate {a.serve_rate}/s, mixed precision")
+        sopt = gfc.Options(precision=gfc.PREC_MIXED)
         with gfc.Server(ctx, max_jobs=64) as srv:
-            serve_out, _ = serve.run_stream(srv, reqs, gfc.Options(precision=gfc.PREC_MIXED),
-                                            log=log)
+            serve_out, comps = serve.run_stream(srv, reqs, sopt, log=log)
+        # offered load against the engine's lone-request service times
+        lone = serve.lone_service_times(ctx, reqs, sopt)
+        serve_out.update(serve.utilisation(reqs, comps, lone))
+        serve_out["lone_service_ms"] = {f"{k[0]}^3x{k[1]} tol {k[2]:.0e}": round(v[0] * 1e3, 2)
+                                        for k, v in sorted(lone.items())}
         serve_out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in serve_out.items()}
         serve_out["workload"] = ("config 4: sizes {16^3x32: 40%, 24^3x48: 30%, 32^3x64: 20%, "
                                  "48^3x96: 10%}, eps U[0.2,0.4], tol {1e-14, 1e-16}, "

@@ -5,7 +5,9 @@ from a weighted set, eps uniform in a range, tolerance from a set; every
 request is an independent seeded smooth configuration.  `run_stream` replays
 the arrival times in real time against a server, polling it between arrivals,
 and measures each request's latency from its scheduled arrival to the moment
-its gauge transformation is materialised in HBM.
+its gauge transformation is materialised in HBM.  `lone_service_times`
+measures what each request class costs alone on the idle GPU, from which
+`utilisation` gives the offered load rho = lambda E[S].
 """
 from __future__ import annotations
 
@@ -14,7 +16,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import Geometry, Options, Params, PREC_MIXED, Server
+from . import Geometry, Job, Options, Params, PREC_MIXED, Server
 
 CONFIG4_SIZES = ((16, 32, 0.4), (24, 48, 0.3), (32, 64, 0.2), (48, 96, 0.1))
 
@@ -92,3 +94,53 @@ def run_stream(server: Server, reqs, options: Options | None = None, log=None):
         "chunks": st["chunks"], "graph_replays": st["graph_replays"],
         "graphs_built": st["graphs_built"],
     }, comps
+
+
+def lone_service_times(ctx, reqs, options: Options | None = None, reps: int = 2):
+    """Service time of each (lattice size, tolerance) class of reqs when its fix
+    runs alone on the idle GPU: gfc_job_fix of the class's first request
+    (after one warm-up fix), with the server's per-request options.  Returns
+    {(ns, nt, tolerance): (seconds, sweeps)}."""
+    options = options or Options(precision=PREC_MIXED)
+    out = {}
+    for r in reqs:
+        key = (r.geom.ns, r.geom.nt, r.tolerance)
+        if key in out:
+            continue
+        with Job(ctx, r.geom, options.precision) as job:
+            job.generate_smooth(r.seed, r.eps)
+            job.reunitarize()
+            job.save_config()
+            best = None
+            for _ in range(reps):
+                job.restore_config()
+                t = time.perf_counter()
+                rc, rep = job.fix(request_params(r), options)
+                dt = time.perf_counter() - t
+                best = dt if best is None else min(best, dt)
+            out[key] = (best, rep.sweeps)
+    return out
+
+
+def utilisation(reqs, comps, lone) -> dict:
+    """Offered load of the stream against the engine's lone-request capacity.
+
+    rho = lambda E[S]: arrivals per second over the stream times the mean of
+    each request's lone service time S (its class's lone fix time, scaled by
+    its own sweep count).  rho < 1 means a server that ran one request at a
+    time at the lone speed would keep up; the scheduler also overlaps
+    requests, so it can sustain somewhat more.  capacity_site_updates_per_s is
+    the lone engine's rate on this mix: the site updates the requests needed
+    over the sum of their lone service times."""
+    span = max(reqs[-1].t_arrival - reqs[0].t_arrival, 1e-9)
+    lam = (len(reqs) - 1) / span
+    S, work = [], 0.0
+    for c in comps:
+        r = reqs[c.tag]
+        sec, sweeps = lone[(r.geom.ns, r.geom.nt, r.tolerance)]
+        S.append(sec * c.sweeps / max(sweeps, 1))
+        work += c.sweeps * (c.ns ** 3 * c.nt // 2)
+    ES = float(np.mean(S))
+    return {"rho": lam * ES, "arrival_rate_per_s": lam, "mean_lone_service_ms": ES * 1e3,
+            "offered_site_updates_per_s": work / span,
+            "capacity_site_updates_per_s": work / float(np.sum(S))}
