@@ -12,9 +12,14 @@ value        site updates per second of the FP64 drop-in path
                (half-sweeps x V/2 / wall time of the step)
 fp64_seconds wall time to e2 < 1e-16, FP64 path
 mixed_*      the FP32 -> FP64 path on the same input and schedule
-roofline     the FP64 sweep kernel: algorithmic bytes (6 spatial links + G(n),
-             read and written once: 2016 B per site update) over its mean
-             duration measured with HIP events on the engine's stream
+roofline     the kernel of the timed fix, k_sweep<double> in cache-resident
+             timeslice waves: the factored update's algorithmic bytes (864 B
+             per site update: two stored rows of the 6 spatial u0 links, G(n)
+             read and written, one neighbour G per update) per lattice
+             half-sweep over the half-sweep's duration measured with HIP
+             events on the job's stream (fork to join of the waves), and the
+             copy-calibrated L2-miss traffic of the same launches from the
+             committed rocprofv3 counters (profiles/sweep_fix_48x96_pmc.txt)
 cpu_baseline the reference's OpenMP CPU path (compiled from its sources into
              oracle/_ref) timed on this host on a bounded sample of the same
              workload: a few half-sweeps of the same lattice
@@ -34,23 +39,40 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gaugefix-coulomb_amd"))
 
-BYTES_PER_UPDATE_FP64 = 2016  # 14 x 144 B
-BYTES_PER_UPDATE_FP32 = 1008
+# bytes per site update credited to the reference's update: 6 spatial links
+# and G(n), full 3x3 matrices read and written once (14 x 144 B); only used
+# for reference-equivalent rates
+REF_BYTES_PER_UPDATE_FP64 = 2016
+REF_BYTES_PER_UPDATE_FP32 = 1008
+# the factored update's algorithmic bytes: two stored rows (96 B in FP64) of
+# the 6 spatial u0 links, G(n) read and written, and one neighbour G (each of
+# the 6 neighbour G read by an update is shared by 6 updates)
+ALG_BYTES_PER_UPDATE_FP64 = 6 * 96 + 96 + 96 + 96
+ALG_BYTES_PER_UPDATE_FP32 = ALG_BYTES_PER_UPDATE_FP64 // 2
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, a wave64 FP64 (or unpacked FP32)
+# instruction every 4 cycles at 2.4 GHz
+VALU_WAVE_INSTR_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(ns: int, nt: int, tag: str):
-    """HBM bytes per launch of a sweep kernel from the committed rocprofv3 PMC
-    summary for this lattice (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", f"sweep_{ns}x{nt}_pmc.txt")
+def pmc_value(name: str, key: str):
+    """A figure from a committed rocprofv3 PMC summary (profiles/<name>)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    m = re.search(tag + r"_hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(key + r":\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
+
+
+def pmc_traffic(ns: int, nt: int, tag: str):
+    """L2-miss bytes per whole-lattice launch of a sweep kernel from the
+    committed rocprofv3 PMC summary for this lattice (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated)."""
+    return pmc_value(f"sweep_{ns}x{nt}_pmc.txt", tag + "_hbm_bytes_per_launch")
 
 
 class _CStdoutToStderr:
@@ -267,18 +289,21 @@ def config5(ctx, ns: int, nt: int, eps: float, seed: int):
             out["fp64"]["seconds"] / out["mixed_retire"]["seconds"], 3)
         fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
                            max_sweeps_to_fix=40, sweeps_to_reunitarization=10 ** 6)
-        for name, opt, bpu in (("fp64", gfc.PREC_FP64, BYTES_PER_UPDATE_FP64),
-                               ("fp32", gfc.PREC_MIXED, BYTES_PER_UPDATE_FP32)):
+        for name, opt, bpu in (("fp64", gfc.PREC_FP64, ALG_BYTES_PER_UPDATE_FP64),
+                               ("fp32", gfc.PREC_MIXED, ALG_BYTES_PER_UPDATE_FP32)):
             job.restore_config()
             rc, rep = job.fix(fixed, gfc.Options(precision=opt, mixed_switch=0.0,
                                                  mixed_probe_every=10 ** 6, time_sweeps=1,
                                                  slice_group=-1))
             ms = rep.seconds_sweeps / rep.sweeps * 1e3
             gbs = bpu * vh / (ms * 1e-3) / 1e9
+            traffic = pmc_traffic(ns, nt, name)
             out[f"{name}_sweep_kernel_ms"] = round(ms, 4)
             out[f"{name}_sweep_GBs_algorithmic"] = round(gbs, 1)
             out[f"{name}_sweep_frac_of_hbm_peak"] = round(gbs / HBM_PEAK_GBS, 4)
-            out[f"{name}_sweep_hbm_bytes_per_launch"] = pmc_traffic(ns, nt, name)
+            out[f"{name}_sweep_hbm_bytes_per_launch"] = traffic
+            if traffic:
+                out[f"{name}_sweep_traffic_GBs"] = round(traffic / (ms * 1e-3) / 1e9, 1)
             log(f"config5 {name} sweep kernel: {ms:.4f} ms/launch, {gbs:.1f} GB/s algorithmic")
     return out
 
@@ -380,14 +405,23 @@ def main():
     log(f"fp64: {a.steps} steps, sweeps={sweeps}, {step_s:.3f}s/step, {value:.4e} upd/s, "
         f"e2={e2_64:.3e}")
 
-    # sweep-kernel duration with HIP events on the engine stream (untimed run),
-    # one whole-lattice launch per half-sweep
+    # the timed fix's sweep kernels: HIP events on the job's stream bracket
+    # every timeslice-blocked run of half-sweeps (fork of the waves to their
+    # join), so seconds_sweeps / sweeps is the duration of one lattice
+    # half-sweep as the timed region runs it (untimed repeat of the same fix)
+    job.restore_config()
+    rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1))
+    half_ms = rep.seconds_sweeps / rep.sweeps * 1e3
+    achieved = ALG_BYTES_PER_UPDATE_FP64 * vh / (half_ms * 1e-3) / 1e9
+    traffic_upd = pmc_value("sweep_fix_48x96_pmc.txt", "fix_fp64_traffic_bytes_per_update")
+    valu_upd = pmc_value("sweep_fix_48x96_pmc.txt", "fix_fp64_valu_instructions_per_update")
+    log(f"fp64 fix sweeps: {half_ms:.4f} ms per half-sweep, {achieved:.1f} GB/s algorithmic")
+    # one whole-lattice launch per half-sweep (streaming loads), for comparison
     job.restore_config()
     rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1,
                                           slice_group=-1))
     sweep_ms = rep.seconds_sweeps / rep.sweeps * 1e3
-    achieved = BYTES_PER_UPDATE_FP64 * vh / (sweep_ms * 1e-3) / 1e9
-    log(f"fp64 sweep kernel: {sweep_ms:.4f} ms/launch, {achieved:.1f} GB/s algorithmic")
+    log(f"fp64 whole-lattice sweep kernel: {sweep_ms:.4f} ms/launch")
 
     def time_fix(prm, opt, reps):
         ts, last = [], None
@@ -431,7 +465,8 @@ def main():
                                         "sweeps_fp32": rep_i.sweeps_fp32,
                                         "speedup_vs_fp64": round(step_s / imm_s, 3)},
             "fp32_sweep_kernel_ms": round(sweep32_ms, 5),
-            "fp32_sweep_GBs_algorithmic": round(BYTES_PER_UPDATE_FP32 * vh / sweep32_ms / 1e6, 1),
+            "fp32_sweep_GBs_algorithmic": round(ALG_BYTES_PER_UPDATE_FP32 * vh / sweep32_ms / 1e6,
+                                                1),
             "dense_schedule": {"fp64_seconds": round(d64_s, 4), "fp64_sweeps": d64.sweeps,
                                "mixed_seconds": round(dmx_s, 4), "mixed_sweeps": dmx.sweeps,
                                "mixed_sweeps_fp32": dmx.sweeps_fp32,
@@ -502,16 +537,28 @@ def main():
         "fp64_final_e2": e2_64,
         "fp64_F": F64,
         "fp64_plaquette_drift": max(abs(plaq64[0] - plaq0[0]), abs(plaq64[1] - plaq0[1])),
-        "sweep_kernel_ms": round(sweep_ms, 5),
-        # the fix itself runs cache-resident timeslice waves: its own rate in
-        # algorithmic bytes (2016 B per site update)
-        "fix_algorithmic_GBs": round(value * 2016 / 1e9, 1),
+        "fix_half_sweep_ms": round(half_ms, 5),
+        "whole_lattice_sweep_kernel_ms": round(sweep_ms, 5),
+        "whole_lattice_sweep_GBs_algorithmic": round(
+            ALG_BYTES_PER_UPDATE_FP64 * vh / (sweep_ms * 1e-3) / 1e9, 1),
+        # the fix's rate in the bytes the reference's own update moves (2016 B
+        # per site update): a reference-equivalent rate, not a memory rate
+        "fix_reference_equivalent_GBs": round(value * REF_BYTES_PER_UPDATE_FP64 / 1e9, 1),
         **out_mixed,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic(a.ns, a.nt, "fp64")},
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(traffic_upd * vh) if traffic_upd else None,
+            "kernel": "k_sweep<double> of the timed fix: 96 single-timeslice launches of "
+                      "55296 site updates per half-sweep, cache-resident waves on 4 streams",
+            "per": "lattice half-sweep (5308416 site updates)",
+            "algorithmic_bytes_per_update": ALG_BYTES_PER_UPDATE_FP64,
+            "traffic_bytes_per_update": traffic_upd,
+            "traffic_GBs": round(traffic_upd * vh / (half_ms * 1e-3) / 1e9, 1)
+            if traffic_upd else None,
+            "valu_issue_frac": round(valu_upd * vh / 64 / (half_ms * 1e-3) / VALU_WAVE_INSTR_PEAK,
+                                     4) if valu_upd else None},
         "copy_kernel_GBs": round(copy_gbs, 1),
-        "sweep_frac_of_copy": round(achieved / copy_gbs, 4),
         "serve": serve_out,
         "config5": c5,
         "io": io,
