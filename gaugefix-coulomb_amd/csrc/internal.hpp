@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "gfc.h"
 #include "handoff.hpp"
@@ -95,5 +96,24 @@ int promote_g_only(gfc_job *j, hipStream_t s);
 int generate_smooth_device(gfc_job *j, uint64_t seed, double eps);
 int generate_smooth_device(gfc_job *j, hipStream_t s, uint64_t seed, double eps);
 int ensure_partials(gfc_ctx *c, size_t blocks);
+
+// Streams over which timeslice groups run at once; lane[0] is the stream the
+// work is forked from and joined back into.
+struct StreamSet {
+    hipStream_t lane[gfc_ctx::kLanes] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[gfc_ctx::kLanes] = {};
+};
+StreamSet ctx_streams(gfc_ctx *c);
+// Timeslices per group (0: whole-lattice sweeps) and streams for the sweeps
+// of precision fp32 under options o (gfc_options.slice_group, slice_lanes).
+int group_size(const gfc_job *j, const gfc_options &o, bool fp32);
+int lane_count(const gfc_job *j, const gfc_options &o, bool fp32);
+// Half-sweeps [from, to) timeslice group by timeslice group on the streams of
+// ss, with the reunitarization of G after every sweep s + 1 < to that is a
+// multiple of R folded in; bit-identical to whole-lattice sweeps.
+int sweep_blocked(gfc_job *j, const StreamSet &ss, bool fp32, int from, int to, int group,
+                  int max_lanes, double omega, int hits, unsigned R,
+                  const std::vector<char> *active = nullptr);
 
 }  // namespace gfcx

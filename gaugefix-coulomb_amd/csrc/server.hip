@@ -12,6 +12,8 @@
 // sites, and larger), one stream each, so that a small lattice is never
 // stepped in lockstep with a large one: its half-sweeps take microseconds
 // while a large lattice's take a millisecond, and the streams share the GPU.
+// The large lane runs each lattice's half-sweeps as gfc_job_fix does, in
+// cache-resident timeslice waves over its own set of four streams.
 // A lane's poll:
 //   1. admits the queued requests of its size class that fit;
 //   2. one batched residue evaluation for every lattice at a check, a mixed-
@@ -24,7 +26,10 @@
 //      max), and every half-sweep step is ONE launch per precision that packs
 //      the 256-site chunks of the lane's lattices (k_sweep_batch).  The chunk's
 //      launch sequence is captured in a HIP graph and replayed while the plan
-//      repeats.  At most two chunks of a lane are in flight.
+//      repeats.  In the large lane each lattice instead runs up to its next
+//      check, probe or hand-off (at most kWaveChunk half-sweeps) in timeslice
+//      waves, with its reunitarizations folded in.  At most two chunks of a
+//      lane are in flight.
 // gfc_server_poll serves, in turn, every lane that can proceed without
 // waiting for the GPU, so a busy lane never holds up the others.
 // Per-site arithmetic and reduction order are those of the single-lattice
@@ -60,6 +65,12 @@ constexpr int kLanes = gfc_ctx::kLanes;
 // that its data stays in the cache from one half-sweep to the next; larger
 // lanes stream with non-temporal loads.
 constexpr int64_t kLaneCacheBytes = 96ll << 20;
+
+// The lane of the largest lattices, which sweeps in timeslice waves, and the
+// most half-sweeps of one lattice in one of its chunks (so that a request
+// queued behind it is admitted within about a tenth of a second).
+constexpr int kWaveLane = gfc_ctx::kLanes - 1;
+constexpr int kWaveChunk = 100;
 
 // Size class of a lattice: its lane.
 int lane_of(const gfc_geom &g) {
@@ -125,6 +136,9 @@ struct Lane {
     std::vector<int64_t> plan_sig;
     hipEvent_t chunk_ev[2] = {nullptr, nullptr};
     int64_t chunks = 0;
+    // timeslice-wave lane: st and three streams of its own, default priority
+    bool waves = false;
+    gfcx::StreamSet wss;
 };
 
 }  // namespace
@@ -343,7 +357,51 @@ void fail_request(Req &r, int code) {
 
 int next_multiple(int s, int m) { return (s / m + 1) * m; }
 
+// A chunk of the timeslice-wave lane: each running lattice sweeps up to its
+// next check, probe, hand-off or max sweeps (at most kWaveChunk half-sweeps)
+// in timeslice waves on the lane's streams, one lattice after the other.
+int run_waves(gfc_server *s, Lane &L) {
+    std::vector<Req *> run;
+    std::vector<int> stops;
+    for (auto &a : L.active) {
+        if (a->state != RUN) continue;
+        Req &r = *a;
+        int64_t stop = (int64_t)r.sweep + kWaveChunk;
+        if (r.due > r.sweep) stop = std::min<int64_t>(stop, r.due);
+        if ((int64_t)r.p.max_sweeps_to_fix > r.sweep)
+            stop = std::min<int64_t>(stop, (int64_t)r.p.max_sweeps_to_fix);
+        if (r.fp32) {
+            if (!r.ho.at_floor) stop = std::min(stop, (int64_t)next_multiple(r.sweep, r.o.mixed_probe_every));
+            if (r.ho.switch_at > r.sweep) stop = std::min<int64_t>(stop, r.ho.switch_at);
+        }
+        run.push_back(&r);
+        stops.push_back((int)stop);
+    }
+    if (run.empty()) return GFC_OK;
+    hipEvent_t &ev = L.chunk_ev[L.chunks % 2];
+    HIP_TRY(hipEventSynchronize(ev));
+    for (size_t q = 0; q < run.size(); q++) {
+        Req &r = *run[q];
+        int group = gfcx::group_size(r.job, r.o, r.fp32);
+        if (group <= 0) group = r.job->geom.nt;  // fits the cache whole: one group
+        RC_TRY(gfcx::sweep_blocked(r.job, L.wss, r.fp32, r.sweep, stops[q], group,
+                                   gfcx::lane_count(r.job, r.o, r.fp32), (double)r.p.omega_OR,
+                                   (int)r.p.hits, r.p.sweeps_to_reunitarization));
+    }
+    HIP_TRY(hipEventRecord(ev, L.st));
+    L.chunks++;
+    s->stats.chunks++;
+    for (size_t q = 0; q < run.size(); q++) {
+        const int n = stops[q] - run[q]->sweep;
+        run[q]->sweep = stops[q];
+        if (run[q]->fp32) run[q]->sweeps_fp32 += n;
+        s->stats.site_updates += (int64_t)n * run[q]->job->Vh;
+    }
+    return GFC_OK;
+}
+
 int plan_and_run(gfc_server *s, Lane &L) {
+    if (L.waves) return run_waves(s, L);
     // stop points: next reunitarization point, next check, next probe,
     // hand-off, max
     std::vector<Req *> run;
@@ -641,6 +699,15 @@ void free_lane(Lane &L) {
     }
     for (hipEvent_t e : L.chunk_ev)
         if (e) hipEventDestroy(e);
+    // the timeslice-wave lane's own streams (lane 0 of the set is L.st)
+    for (int l = 1; l < gfc_ctx::kLanes; l++)
+        if (L.wss.lane[l]) {
+            hipStreamSynchronize(L.wss.lane[l]);
+            hipStreamDestroy(L.wss.lane[l]);
+        }
+    if (L.wss.fork) hipEventDestroy(L.wss.fork);
+    for (int l = 0; l < gfc_ctx::kLanes; l++)
+        if (L.wss.join[l]) hipEventDestroy(L.wss.join[l]);
     for (auto &a : L.active) gfc_job_destroy(a->job);
     hipFree(L.d_jobs);
     hipFree(L.d_oitems);
@@ -695,6 +762,15 @@ int gfc_server_create(gfc_ctx *ctx, int64_t mem_budget, int max_jobs, int keep_r
         for (int k = 0; k < 2; k++) {
             HIP_TRY(hipEventCreateWithFlags(&L.chunk_ev[k], hipEventDisableTiming));
             HIP_TRY(hipEventRecord(L.chunk_ev[k], L.st));
+        }
+        if (li == kWaveLane) {
+            L.waves = true;
+            L.wss.lane[0] = L.st;
+            for (int l = 1; l < gfc_ctx::kLanes; l++)
+                HIP_TRY(hipStreamCreateWithFlags(&L.wss.lane[l], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&L.wss.fork, hipEventDisableTiming));
+            for (int l = 0; l < gfc_ctx::kLanes; l++)
+                HIP_TRY(hipEventCreateWithFlags(&L.wss.join[l], hipEventDisableTiming));
         }
     }
     *out = s.release();
