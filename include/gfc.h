@@ -156,17 +156,21 @@ typedef struct {
                                groups, so that kernel tails overlap.
                                Timeslices do not interact, so the result is
                                bit-identical to whole-lattice sweeps.
-                               0 (default): groups sized so that a wave of
-                               four stays in 192 MiB of the Infinity Cache
-                               between half-sweeps (read with cached loads),
-                               whole-lattice sweeps for lattices that fit in
-                               it, single timeslices when a wave of them does
-                               not fit; -1: whole-lattice sweeps.  With
-                               retire_slices, one timeslice per group.        */
+                               0 (default): groups sized so that a wave
+                               (one group per stream) stays in 192 MiB of the
+                               Infinity Cache between half-sweeps (read with
+                               cached loads); a lattice that fits in it whole
+                               is swept as three groups of nt/3 timeslices
+                               when it has at least 2^17 sites per parity,
+                               whole otherwise; single timeslices when a wave
+                               of them does not fit; -1: whole-lattice
+                               sweeps.  With retire_slices, one timeslice per
+                               group.                                         */
     int slice_lanes;        /* streams over which the timeslice groups run
-                               concurrently (a wave holds one group per
-                               stream): 1-4, or 0 (default) for the measured
-                               best per precision                             */
+                               concurrently: 1-4, or 0 (default): three for a
+                               lattice that fits the cache whole, two for
+                               FP64 timeslices too large for a cache-resident
+                               wave of four, four otherwise                   */
 } gfc_options;
 
 typedef struct {
