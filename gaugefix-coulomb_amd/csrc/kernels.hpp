@@ -115,6 +115,15 @@ __device__ __forceinline__ void load_su3(mat3<T> &m, const cplx<T> *__restrict__
     m_fill_row2(m);
 }
 
+// The two stored rows only (row 2 left unset).
+template <bool STREAM, typename T>
+__device__ __forceinline__ void load_rows01(mat3<T> &m, const cplx<T> *__restrict__ field, int h) {
+    const uint32_t off = tile_bytes<T>(h);
+#pragma unroll
+    for (int k = 0; k < kPlanes; k++)
+        m.e[k] = STREAM ? ld_stream(elem_ptr(field, off, k)) : *elem_ptr(field, off, k);
+}
+
 template <bool STREAM, typename T>
 __device__ __forceinline__ void store_su3(const mat3<T> &m, cplx<T> *__restrict__ field, int h) {
     const uint32_t off = tile_bytes<T>(h);
@@ -194,17 +203,19 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
     const Site s = site_of(h, p, ns);
     const Nbr nb = neighbours(s, h, ns);
     const cplx<T> *gq = g_field(static_cast<const cplx<T> *>(g), Vh, 1 - p);  // other parity
+    // the six products are SU(3): rows 0, 1 from the stored rows of the left
+    // factor, row 2 by closure (m_mac_nd_su3, m_mac_dagger_su3)
     mat3<T> W;
     m_zero(W);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         mat3<T> uf, gf, ub, gb;
-        load_su3<STREAM>(uf, u0_field(u0, Vh, p, i), h);
+        load_rows01<STREAM>(uf, u0_field(u0, Vh, p, i), h);
         load_su3<false>(gf, gq, nb.fwd[i]);
+        m_mac_nd_su3(W, uf, gf);  // u0_i(n) G(n+i)^dagger
         load_su3<STREAM>(ub, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
-        load_su3<false>(gb, gq, nb.bwd[i]);
-        m_mac_nd(W, uf, gf);  // u0_i(n) G(n+i)^dagger
-        m_mac_dd(W, ub, gb);  // u0_i(n-i)^dagger G(n-i)^dagger
+        load_rows01<false>(gb, gq, nb.bwd[i]);
+        m_mac_dagger_su3(W, gb, ub);  // (G(n-i) u0_i(n-i))^dagger
     }
     cplx<T> *gp = g_field(g, Vh, p);
     mat3<T> gn, w, upd, out;
