@@ -493,22 +493,31 @@ int lane_count(const gfc_job *j, const gfc_options &o, bool fp32) {
     return gfc_ctx::kLanes;
 }
 
+template <typename T, bool POUT, bool STREAM>
+void launch_sweep_pin(const cplx<T> *u0, cplx<T> *g, hipStream_t s, int ns, int Vh, int p,
+                      T omega, int hits, int h0, int count, bool pin) {
+    const unsigned b = nblocks(count);
+    if (pin)
+        hipLaunchKernelGGL((gfc::k_sweep<T, POUT, STREAM, true>), dim3(b), dim3(kBlock), 0, s,
+                           u0, g, ns, Vh, p, omega, hits, h0, count);
+    else
+        hipLaunchKernelGGL((gfc::k_sweep<T, POUT, STREAM, false>), dim3(b), dim3(kBlock), 0, s,
+                           u0, g, ns, Vh, p, omega, hits, h0, count);
+}
+
+// A half-sweep launch: pout projects the swept parity's new G on output, pin
+// its G on input (see sweep_site), stream selects non-temporal loads.
 template <typename T>
 int launch_sweep(const cplx<T> *u0, cplx<T> *g, hipStream_t s, int ns, int Vh, int p, T omega,
-                 int hits, int h0, int count, bool pout, bool stream) {
-    const unsigned b = nblocks(count);
+                 int hits, int h0, int count, bool pout, bool stream, bool pin = false) {
     if (pout && stream)
-        hipLaunchKernelGGL((gfc::k_sweep<T, true, true>), dim3(b), dim3(kBlock), 0, s, u0, g,
-                           ns, Vh, p, omega, hits, h0, count);
+        launch_sweep_pin<T, true, true>(u0, g, s, ns, Vh, p, omega, hits, h0, count, pin);
     else if (pout)
-        hipLaunchKernelGGL((gfc::k_sweep<T, true, false>), dim3(b), dim3(kBlock), 0, s, u0, g,
-                           ns, Vh, p, omega, hits, h0, count);
+        launch_sweep_pin<T, true, false>(u0, g, s, ns, Vh, p, omega, hits, h0, count, pin);
     else if (stream)
-        hipLaunchKernelGGL((gfc::k_sweep<T, false, true>), dim3(b), dim3(kBlock), 0, s, u0, g,
-                           ns, Vh, p, omega, hits, h0, count);
+        launch_sweep_pin<T, false, true>(u0, g, s, ns, Vh, p, omega, hits, h0, count, pin);
     else
-        hipLaunchKernelGGL((gfc::k_sweep<T, false, false>), dim3(b), dim3(kBlock), 0, s, u0, g,
-                           ns, Vh, p, omega, hits, h0, count);
+        launch_sweep_pin<T, false, false>(u0, g, s, ns, Vh, p, omega, hits, h0, count, pin);
     LAUNCH_CHECK();
     return GFC_OK;
 }
@@ -521,20 +530,6 @@ int half_sweep(gfc_job *j, SweepState &st, int parity, double omega, int hits) {
         return launch_sweep(j->u0f, j->gf, s, ns, Vh, parity, (float)omega, hits, 0, Vh, false,
                             stream);
     return launch_sweep(j->u0, j->g, s, ns, Vh, parity, omega, hits, 0, Vh, false, stream);
-}
-
-// Projects G on parity indices [h0, h0 + count) of parities [p0, p0 + np)
-// of the live array.
-int reunit_range(gfc_job *j, hipStream_t s, bool fp32, int p0, int np, int h0, int count) {
-    const unsigned b = nblocks((int64_t)np * count);
-    if (fp32)
-        hipLaunchKernelGGL(gfc::k_g_reunit<float>, dim3(b), dim3(kBlock), 0, s, j->gf, j->Vh, p0,
-                           np, h0, count);
-    else
-        hipLaunchKernelGGL(gfc::k_g_reunit<double>, dim3(b), dim3(kBlock), 0, s, j->g, j->Vh, p0,
-                           np, h0, count);
-    LAUNCH_CHECK();
-    return GFC_OK;
 }
 
 // Projects G on parities [p0, p0 + np) of the live array.
@@ -590,10 +585,11 @@ StreamSet ctx_streams(gfc_ctx *c) {
 
 // Half-sweeps [from, to) (parity = sweep number & 1), timeslice group by
 // timeslice group, with the reunitarization of G after every sweep number
-// s + 1 < to that is a multiple of R (folded into the sweeps around it).
-// Every site update reads only sites of its own timeslice, so each group runs
-// through the sweeps on its own and the result equals that of whole-lattice
-// sweeps bit for bit.
+// s + 1 < to that is a multiple of R folded into the two sweeps around it:
+// sweep s projects its own parity's G on output, sweep s + 1 the other
+// parity's on input.  Every site update reads only sites of its own
+// timeslice, so each group runs through the sweeps on its own and the result
+// equals that of whole-lattice sweeps bit for bit.
 int sweep_blocked(gfc_job *j, const StreamSet &ss, bool fp32, int from, int to, int group,
                   int max_lanes, double omega, int hits, unsigned R,
                   const std::vector<char> *active) {
@@ -618,9 +614,10 @@ int sweep_blocked(gfc_job *j, const StreamSet &ss, bool fp32, int from, int to, 
         for (int w0 = 0; w0 < ng; w0 += lanes) {
             const int w1 = std::min(ng, w0 + lanes);
             for (int k = from; k < to; k++) {
-                // a reunitarization after sweep k + 1 < to: sweep k projects its
-                // own parity on output, a pass over the group the other parity
+                // a reunitarization after sweep number k + 1 < to: sweep k projects
+                // its own parity on output, sweep k + 1 the other parity on input
                 const bool pout = k + 1 < to && (unsigned)(k + 1) % R == 0;
+                const bool pin = k > from && (unsigned)k % R == 0;
                 for (int wi = w0; wi < w1; wi++) {
                     hipStream_t s = ss.lane[wi - w0];
                     const int t0 = groups[wi] * group;
@@ -628,11 +625,10 @@ int sweep_blocked(gfc_job *j, const StreamSet &ss, bool fp32, int from, int to, 
                     const int count = (t0 + group <= nt ? group : nt - t0) * S;
                     if (fp32)
                         RC_TRY(launch_sweep(j->u0f, j->gf, s, ns, Vh, k & 1, (float)omega, hits,
-                                            h0, count, pout, stream));
+                                            h0, count, pout, stream, pin));
                     else
                         RC_TRY(launch_sweep(j->u0, j->g, s, ns, Vh, k & 1, omega, hits, h0,
-                                            count, pout, stream));
-                    if (pout) RC_TRY(reunit_range(j, s, fp32, 1 - (k & 1), 1, h0, count));
+                                            count, pout, stream, pin));
                 }
             }
         }

@@ -192,17 +192,30 @@ __device__ __forceinline__ Nbr neighbours(const Site &s, int h, int ns) {
 // batched (serving) kernels, so a site computes the same bits in both.
 // POUT projects the new G(n) onto SU(3) (in FP64, project_g) before storing
 // it: the same bits as a reunitarization of this parity's G right after the
-// sweep, since the projection reads rows 0 and 1 only.  STREAM reads u0 and G(n) with
-// non-temporal loads (data read once per sweep); without it they are cached,
-// for sweeps whose data stays in the Infinity Cache from one half-sweep to the
-// next.
-template <typename T, bool POUT = false, bool STREAM = true>
+// sweep, since the projection reads rows 0 and 1 only.  PIN projects G(n)
+// right after loading it and rebuilds its third row from the projected rows,
+// as a store and reload would: the same bits as a reunitarization of this
+// parity's G before the sweep (nothing else reads it in between).  STREAM
+// reads u0 and G(n) with non-temporal loads (data read once per sweep);
+// without it they are cached, for sweeps whose data stays in the Infinity
+// Cache from one half-sweep to the next.
+template <typename T, bool POUT = false, bool STREAM = true, bool PIN = false>
 __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
                                            cplx<T> *__restrict__ g, int ns, int Vh, int p,
                                            T omega, int hits, int h) {
     const Site s = site_of(h, p, ns);
     const Nbr nb = neighbours(s, h, ns);
     const cplx<T> *gq = g_field(static_cast<const cplx<T> *>(g), Vh, 1 - p);  // other parity
+    cplx<T> *gp = g_field(g, Vh, p);
+    // with PIN, G(n) is loaded and projected first and its rows 0 and 1 kept
+    // through the W products (loaded later, next to its use, the projection
+    // makes the compiler hold twice the registers)
+    mat3<T> gn;
+    if (PIN) {
+        load_rows01<STREAM>(gn, static_cast<const cplx<T> *>(gp), h);
+        project_g(gn);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     // the six products are SU(3): rows 0, 1 from the stored rows of the left
     // factor, row 2 by closure (m_mac_nd_su3, m_mac_dagger_su3)
     mat3<T> W;
@@ -217,9 +230,11 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
         load_rows01<false>(gb, gq, nb.bwd[i]);
         m_mac_dagger_su3(W, gb, ub);  // (G(n-i) u0_i(n-i))^dagger
     }
-    cplx<T> *gp = g_field(g, Vh, p);
-    mat3<T> gn, w, upd, out;
-    load_su3<STREAM>(gn, static_cast<const cplx<T> *>(gp), h);
+    if (PIN)
+        m_fill_row2(gn);  // the third row a store and reload would give
+    else
+        load_su3<STREAM>(gn, static_cast<const cplx<T> *>(gp), h);
+    mat3<T> w, upd, out;
     m_mul(gn, W, w);
     la_or_update(w, hits, omega, upd);
     m_mul(upd, gn, out);
@@ -248,7 +263,7 @@ __device__ __forceinline__ unsigned xcd_order(unsigned i, unsigned n) {
 }
 
 // One red/black half-sweep on parity p over parity indices [h0, h0 + count).
-template <typename T, bool POUT = false, bool STREAM = true>
+template <typename T, bool POUT = false, bool STREAM = true, bool PIN = false>
 __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0,
                                                   cplx<T> *__restrict__ g, int ns, int Vh,
                                                   int p, T omega, int hits, int h0,
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const cplx<T> *__restrict__ u0
     const unsigned b = STREAM ? blockIdx.x : xcd_order(blockIdx.x, gridDim.x);
     const int tid = (int)(b * kBlock + threadIdx.x);
     if (tid >= count) return;
-    sweep_site<T, POUT, STREAM>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
+    sweep_site<T, POUT, STREAM, PIN>(u0, g, ns, Vh, p, omega, hits, h0 + tid);
 }
 
 // ---------------------------------------------------------------------------
