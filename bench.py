@@ -390,13 +390,19 @@ def main():
         job.restore_config()
         rc, rep = job.fix(params, fp64)
         log(f"warmup fp64: sweeps={rc} {rep.seconds:.3f}s")
-    sweeps = []
+    # the timed steps' sweep kernels: HIP events on the job's stream bracket
+    # every timeslice-blocked run of half-sweeps (fork of the waves to their
+    # join), so seconds_sweeps / sweeps is the duration of one lattice
+    # half-sweep as the timed region runs it
+    fp64_timed = gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1)
+    sweeps, sweep_s = [], 0.0
     t0 = time.perf_counter()
     for i in range(a.steps):
         job.restore_config()
-        rc, rep = job.fix(params, fp64)
+        rc, rep = job.fix(params, fp64_timed)
         assert rc > 0, rc
         sweeps.append(rc)
+        sweep_s += rep.seconds_sweeps
     t1 = time.perf_counter()
     step_s = (t1 - t0) / a.steps
     value = sum(sweeps) * vh / (t1 - t0)
@@ -404,14 +410,7 @@ def main():
     plaq64 = job.plaquette()
     log(f"fp64: {a.steps} steps, sweeps={sweeps}, {step_s:.3f}s/step, {value:.4e} upd/s, "
         f"e2={e2_64:.3e}")
-
-    # the timed fix's sweep kernels: HIP events on the job's stream bracket
-    # every timeslice-blocked run of half-sweeps (fork of the waves to their
-    # join), so seconds_sweeps / sweeps is the duration of one lattice
-    # half-sweep as the timed region runs it (untimed repeat of the same fix)
-    job.restore_config()
-    rc, rep = job.fix(params, gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1))
-    half_ms = rep.seconds_sweeps / rep.sweeps * 1e3
+    half_ms = sweep_s / sum(sweeps) * 1e3
     achieved = ALG_BYTES_PER_UPDATE_FP64 * vh / (half_ms * 1e-3) / 1e9
     traffic_upd = pmc_value("sweep_fix_48x96_pmc.txt", "fix_fp64_traffic_bytes_per_update")
     valu_upd = pmc_value("sweep_fix_48x96_pmc.txt", "fix_fp64_valu_instructions_per_update")

@@ -321,7 +321,9 @@ typedef struct {
 
 /* A scheduler on ctx's four streams: lattices are served in four lanes by
  * size (fewer than 2^18, 2^20, 2^22 sites, and larger), one stream each, so
- * that small lattices do not advance at the pace of large ones.  mem_budget
+ * that small lattices do not advance at the pace of large ones; the large
+ * lane sweeps each lattice in the cache-resident timeslice waves of
+ * gfc_job_fix, on three more streams of its own.  mem_budget
  * (bytes, <= 0: 85% of the free HBM) bounds the summed footprints of resident
  * lattices; max_jobs bounds their number; keep_results keeps finished
  * lattices resident until released.  gfc_server_poll never waits for a busy
