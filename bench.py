@@ -193,8 +193,9 @@ def profile_kernels(ctx, job, n: int):
         f"{2 * 2 ** 30 / sec / 1e9:.1f} GB/s read+write")
 
 
-def profile_fix(ctx, job, n: int):
-    """n FP64 half-sweeps run as the timed fix runs them (default grouping:
+def profile_fix(ctx, job, n: int, fp32: bool = False):
+    """n FP64 (or, with fp32, FP32: the mixed path's first phase, no hand-off)
+    half-sweeps run as the timed fix runs them (default grouping:
     cache-resident timeslice waves on several streams, reunitarization folded
     in every 10 sweeps), then 1 GiB copies: counters for the dominant kernel of
     the timed region and their calibration."""
@@ -202,8 +203,13 @@ def profile_fix(ctx, job, n: int):
     fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=100000000,
                        max_sweeps_to_fix=n, sweeps_to_reunitarization=10)
     job.restore_config()
-    rc, rep = job.fix(fixed, gfc.Options(precision=gfc.PREC_FP64))
-    log(f"profile: {rep.sweeps} FP64 half-sweeps in cache-resident timeslice waves")
+    if fp32:
+        opt = gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0, mixed_probe_every=10 ** 6)
+    else:
+        opt = gfc.Options(precision=gfc.PREC_FP64)
+    rc, rep = job.fix(fixed, opt)
+    log(f"profile: {rep.sweeps} {'FP32' if fp32 else 'FP64'} half-sweeps "
+        f"({rep.sweeps_fp32} in FP32) in cache-resident timeslice waves")
     sec = ctx.copy_probe(2 ** 30, reps=3)
     log(f"profile: copied 1 GiB x 4 with k_copy16 (calibration kernel), "
         f"{2 * 2 ** 30 / sec / 1e9:.1f} GB/s read+write")
@@ -333,6 +339,8 @@ def main():
                     help="only run N FP64 half-sweeps the way the timed fix runs them, plus "
                          "a 1 GiB device copy (rocprofv3 counters of the timed kernels), "
                          "on an input synthesised in HBM, then exit")
+    ap.add_argument("--profile-fp32", action="store_true",
+                    help="with --profile-fix: FP32 half-sweeps (mixed path, no hand-off)")
     ap.add_argument("--device-input", action="store_true",
                     help="with --profile-kernels: synthesise the input in HBM")
     a = ap.parse_args()
@@ -348,11 +356,11 @@ def main():
     vh = V // 2
     if a.profile_fix:
         ctx = gfc.Context(0)
-        with gfc.Job(ctx, geom, gfc.PREC_FP64) as job:
+        with gfc.Job(ctx, geom, gfc.PREC_MIXED if a.profile_fp32 else gfc.PREC_FP64) as job:
             job.generate_smooth(a.seed, a.eps)
             job.reunitarize()
             job.save_config()
-            profile_fix(ctx, job, a.profile_fix)
+            profile_fix(ctx, job, a.profile_fix, a.profile_fp32)
         ctx.close()
         return
     if a.profile_kernels and a.device_input:
@@ -379,6 +387,14 @@ def main():
     copy_s = ctx.copy_probe(2 ** 30, reps=5)
     copy_gbs = 2 * 2 ** 30 / copy_s / 1e9
     log(f"device copy: {copy_gbs:.1f} GB/s (read + write)")
+    # read bandwidth of a working set resident in the Infinity Cache (the
+    # cache budget of the sweeps' waves) and of one in HBM: what the sweeps'
+    # L2-miss traffic runs against
+    cache_read_gbs = max((mib << 20) / ctx.read_probe(mib << 20, reps=50) / 1e9
+                         for mib in (128, 192, 256))
+    hbm_read_gbs = (2 << 30) / ctx.read_probe(2 << 30, reps=10) / 1e9
+    log(f"device read: {cache_read_gbs:.1f} GB/s from a 128-256 MiB set, "
+        f"{hbm_read_gbs:.1f} GB/s from 2 GiB")
     if a.profile_kernels:
         profile_kernels(ctx, job, a.profile_kernels)
         return
@@ -412,8 +428,10 @@ def main():
         f"e2={e2_64:.3e}")
     half_ms = sweep_s / sum(sweeps) * 1e3
     achieved = ALG_BYTES_PER_UPDATE_FP64 * vh / (half_ms * 1e-3) / 1e9
-    traffic_upd = pmc_value("sweep_fix_48x96_pmc.txt", "fix_fp64_traffic_bytes_per_update")
-    valu_upd = pmc_value("sweep_fix_48x96_pmc.txt", "fix_fp64_valu_instructions_per_update")
+    # the committed counters are of the 48^3x96 fix
+    fixprof = f"sweep_fix_{a.ns}x{a.nt}_pmc.txt"
+    traffic_upd = pmc_value(fixprof, "fix_fp64_traffic_bytes_per_update")
+    valu_upd = pmc_value(fixprof, "fix_fp64_valu_instructions_per_update")
     log(f"fp64 fix sweeps: {half_ms:.4f} ms per half-sweep, {achieved:.1f} GB/s algorithmic")
     # one whole-lattice launch per half-sweep (streaming loads), for comparison
     job.restore_config()
@@ -448,6 +466,17 @@ def main():
                                         mixed_probe_every=10 ** 6, time_sweeps=1,
                                         slice_group=-1))
         sweep32_ms = rep32.seconds_sweeps / rep32.sweeps * 1e3
+        # FP32 half-sweeps as the mixed fix runs them (default grouping), HIP
+        # events around each run of blocked sweeps, 200 half-sweeps
+        job.restore_config()
+        rc, rep32f = job.fix(gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
+                                        max_sweeps_to_fix=200, sweeps_to_reunitarization=10),
+                             gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0,
+                                         mixed_probe_every=10 ** 6, time_sweeps=1))
+        half32_ms = rep32f.seconds_sweeps / rep32f.sweeps * 1e3
+        fixprof32 = f"sweep_fix_fp32_{a.ns}x{a.nt}_pmc.txt"
+        t32_upd = pmc_value(fixprof32, "fix_fp32_traffic_bytes_per_update")
+        v32_upd = pmc_value(fixprof32, "fix_fp32_valu_instructions_per_update")
         # the same comparison under the dense check policy (a check every 10 sweeps)
         dense = gfc.Params(tolerance=1e-16, estimate_sweeps_to_gf_progress=0,
                            sweeps_to_reunitarization=10, omega_OR=1.95, hits=2)
@@ -464,6 +493,15 @@ def main():
                                         "sweeps_fp32": rep_i.sweeps_fp32,
                                         "speedup_vs_fp64": round(step_s / imm_s, 3)},
             "fp32_sweep_kernel_ms": round(sweep32_ms, 5),
+            "fix_fp32_half_sweep_ms": round(half32_ms, 5),
+            "fix_fp32_over_fp64_half_sweep": round(half32_ms / half_ms, 4),
+            # the FP32 half-sweep's L2-miss traffic and VALU issue, from the
+            # committed counters of the same FP32 sweeps
+            "fix_fp32_traffic_bytes_per_update": t32_upd,
+            "fix_fp32_traffic_GBs": round(t32_upd * vh / (half32_ms * 1e-3) / 1e9, 1)
+            if t32_upd else None,
+            "fix_fp32_valu_issue_frac": round(v32_upd * vh / 64 / (half32_ms * 1e-3) /
+                                              VALU_WAVE_INSTR_PEAK, 4) if v32_upd else None,
             "fp32_sweep_GBs_algorithmic": round(ALG_BYTES_PER_UPDATE_FP32 * vh / sweep32_ms / 1e6,
                                                 1),
             "dense_schedule": {"fp64_seconds": round(d64_s, 4), "fp64_sweeps": d64.sweeps,
@@ -548,15 +586,24 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": round(traffic_upd * vh) if traffic_upd else None,
-            "kernel": "k_sweep<double> of the timed fix: 96 single-timeslice launches of "
-                      "55296 site updates per half-sweep, cache-resident waves on 4 streams",
-            "per": "lattice half-sweep (5308416 site updates)",
+            "kernel": "k_sweep<double> of the timed fix in its default grouping "
+                      "(48^3x96: 96 single-timeslice launches of 55296 site updates per "
+                      "half-sweep, cache-resident waves on 4 streams)",
+            "per": f"lattice half-sweep ({vh} site updates)",
             "algorithmic_bytes_per_update": ALG_BYTES_PER_UPDATE_FP64,
             "traffic_bytes_per_update": traffic_upd,
             "traffic_GBs": round(traffic_upd * vh / (half_ms * 1e-3) / 1e9, 1)
             if traffic_upd else None,
             "valu_issue_frac": round(valu_upd * vh / 64 / (half_ms * 1e-3) / VALU_WAVE_INSTR_PEAK,
-                                     4) if valu_upd else None},
+                                     4) if valu_upd else None,
+            # the measured read rate of an Infinity-Cache-resident set (best of
+            # 128, 192 and 256 MiB) and of HBM, and the traffic's rate as a
+            # fraction of the former
+            "cache_read_GBs": round(cache_read_gbs, 1),
+            "hbm_read_GBs": round(hbm_read_gbs, 1),
+            "traffic_frac_of_cache_read": round(
+                traffic_upd * vh / (half_ms * 1e-3) / 1e9 / cache_read_gbs, 4)
+            if traffic_upd else None},
         "copy_kernel_GBs": round(copy_gbs, 1),
         "serve": serve_out,
         "config5": c5,

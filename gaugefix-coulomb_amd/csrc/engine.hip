@@ -174,6 +174,45 @@ int gfc_ctx_copy_probe(gfc_ctx *c, int64_t bytes, int reps, double *seconds) {
     return rc;
 }
 
+int gfc_ctx_read_probe(gfc_ctx *c, int64_t bytes, int reps, double *seconds) {
+    if (c == nullptr || bytes < 16 || reps < 1 || seconds == nullptr)
+        return fail(GFC_EINVAL, "gfc_ctx_read_probe: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t n = bytes / 16;
+    const unsigned grid = std::min<unsigned>(nblocks(n), 16384u);
+    double2 *a = nullptr, *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GFC_OK;
+    hipError_t e = hipMalloc(&a, n * 16);
+    if (e == hipSuccess) e = hipMalloc(&sink, (size_t)grid * kBlock * 16);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n * 16, c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        // three passes bring a cache-sized buffer into the Infinity Cache
+        for (int w = 0; w < 3 && e == hipSuccess; w++) {
+            hipLaunchKernelGGL(gfc::k_read16, dim3(grid), dim3(kBlock), 0, c->stream, a, n, sink);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps && e == hipSuccess; r++) {
+            hipLaunchKernelGGL(gfc::k_read16, dim3(grid), dim3(kBlock), 0, c->stream, a, n, sink);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        *seconds = ms * 1e-3 / reps;
+    }
+    if (e != hipSuccess) rc = fail(GFC_EDEVICE, std::string("gfc_ctx_read_probe: ") + hipGetErrorString(e));
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(sink);
+    return rc;
+}
+
 int64_t gfc_job_footprint(const gfc_geom *g, int precision) {
     if (gfc_geom_check(g)) return -1;
     const int64_t V = gfc_geom_volume(g);

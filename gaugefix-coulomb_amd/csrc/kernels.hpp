@@ -666,6 +666,27 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const double2 *__restrict__ i
     if (i < n) out[i] = in[i];
 }
 
+// Reads n 16-byte elements, grid-stride with eight loads in flight per lane,
+// and stores only if their sum takes a value a zeroed buffer cannot give (a
+// read-bandwidth probe: read repeatedly, a buffer of at most 256 MiB stays in
+// the Infinity Cache).
+__global__ __launch_bounds__(kBlock) void k_read16(const double2 *__restrict__ in, int64_t n,
+                                                   double2 *__restrict__ sink) {
+    constexpr int kInFlight = 8;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double2 acc = {0.0, 0.0};
+    for (; i + (kInFlight - 1) * stride < n; i += kInFlight * stride) {
+        double2 v[kInFlight];
+#pragma unroll
+        for (int u = 0; u < kInFlight; u++) v[u] = in[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < kInFlight; u++) acc += v[u];
+    }
+    for (; i < n; i += stride) acc += in[i];
+    if (acc.x == 1.2345e-300) sink[(int64_t)blockIdx.x * kBlock + threadIdx.x] = acc;
+}
+
 // Sums part[3*b + j] over b in fixed order with one block; out[j].
 __device__ __forceinline__ void reduce_parts(const double *__restrict__ part, int nblocks,
                                              double *__restrict__ out) {

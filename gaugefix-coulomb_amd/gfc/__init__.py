@@ -144,6 +144,7 @@ def _load():
         "gfc_ctx_create": (I, [I, C.POINTER(P)]),
         "gfc_ctx_destroy": (None, [P]),
         "gfc_ctx_copy_probe": (I, [P, I64, I, D]),
+        "gfc_ctx_read_probe": (I, [P, I64, I, D]),
         "gfc_job_footprint": (I64, [C.POINTER(_Geom), I]),
         "gfc_job_create": (I, [P, C.POINTER(_Geom), I, C.POINTER(P)]),
         "gfc_job_destroy": (None, [P]),
@@ -366,6 +367,13 @@ class Context:
         """Mean seconds of a device copy of nbytes (streaming 16-byte copy kernel)."""
         s = C.c_double()
         _check(lib.gfc_ctx_copy_probe(self._p, nbytes, reps, C.byref(s)))
+        return s.value
+
+    def read_probe(self, nbytes: int, reps: int = 20) -> float:
+        """Mean seconds of one read pass over a device buffer of nbytes
+        (Infinity-Cache resident up to 256 MiB, HBM above)."""
+        s = C.c_double()
+        _check(lib.gfc_ctx_read_probe(self._p, nbytes, reps, C.byref(s)))
         return s.value
 
     def close(self):

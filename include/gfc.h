@@ -202,6 +202,12 @@ void gfc_ctx_destroy(gfc_ctx *ctx);
  * bandwidth and calibrates memory-traffic counters (no counterpart in the
  * reference). */
 int gfc_ctx_copy_probe(gfc_ctx *ctx, int64_t bytes, int reps, double *seconds);
+/* Reads a zeroed buffer of `bytes` bytes `reps` times (after three untimed
+ * passes) with a 16-byte-per-lane read kernel; *seconds receives the mean
+ * time per pass from HIP events.  A buffer of at most 256 MiB is then read
+ * from the Infinity Cache, a larger one from HBM: the read bandwidth that
+ * bounds the sweeps' L2-miss traffic (no counterpart in the reference). */
+int gfc_ctx_read_probe(gfc_ctx *ctx, int64_t bytes, int reps, double *seconds);
 /* Bytes of device memory a job of this geometry needs for the given
  * precision (used for admission by footprint). */
 int64_t gfc_job_footprint(const gfc_geom *g, int precision);
