@@ -44,11 +44,12 @@ sys.path.insert(0, os.path.join(ROOT, "gaugefix-coulomb_amd"))
 # for reference-equivalent rates
 REF_BYTES_PER_UPDATE_FP64 = 2016
 REF_BYTES_PER_UPDATE_FP32 = 1008
-# the factored update's algorithmic bytes: two stored rows (96 B in FP64) of
-# the 6 spatial u0 links, G(n) read and written, and one neighbour G (each of
-# the 6 neighbour G read by an update is shared by 6 updates)
-ALG_BYTES_PER_UPDATE_FP64 = 6 * 96 + 96 + 96 + 96
-ALG_BYTES_PER_UPDATE_FP32 = ALG_BYTES_PER_UPDATE_FP64 // 2
+# the factored update's algorithmic bytes in the engine's storage: the 6
+# spatial u0 links (FP64: five complex numbers, 80 B; FP32: two rows, 48 B),
+# G(n) read and written and one neighbour G (two rows: 96 B in FP64, 48 B in
+# FP32; each of the 6 neighbour G read by an update is shared by 6 updates)
+ALG_BYTES_PER_UPDATE_FP64 = 6 * 80 + 96 + 96 + 96
+ALG_BYTES_PER_UPDATE_FP32 = 6 * 48 + 48 + 48 + 48
 HBM_PEAK_GBS = 8000.0
 # VALU issue peak: 256 CUs x 4 SIMDs, a wave64 FP64 (or unpacked FP32)
 # instruction every 4 cycles at 2.4 GHz

@@ -216,8 +216,10 @@ int gfc_ctx_read_probe(gfc_ctx *c, int64_t bytes, int reps, double *seconds) {
 int64_t gfc_job_footprint(const gfc_geom *g, int precision) {
     if (gfc_geom_check(g)) return -1;
     const int64_t V = gfc_geom_volume(g);
-    // u0 and G keep two rows (6 complex numbers) per SU(3) matrix, in fields
-    // of whole 32-site tiles per parity and direction
+    // G keeps two rows (6 complex numbers) per SU(3) matrix, in fields of
+    // whole 32-site tiles per parity; the u0 fields are allocated at the same
+    // size per direction (FP64 u0 uses five numbers of them; the allocation
+    // also stages FP32 input)
     const int64_t fs = gfc::field_size(V / 2);
     int64_t b = V * 4 * 144       // cfg
                 + 6 * fs * 16     // u0

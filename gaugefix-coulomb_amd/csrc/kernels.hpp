@@ -87,14 +87,43 @@ __device__ __forceinline__ uint32_t tile_bytes(int h) {
     return tile_elem(h) * (uint32_t)sizeof(cplx<T>);
 }
 
+// u0 fields.  The sweeps read all six u0 links of a site per update, two
+// thirds of their bytes, and they are bound by bytes; so an FP64 u0 matrix
+// keeps five complex numbers: row 0, and the two elements of row 1 that do not
+// sit under row 0's element of largest modulus (its pivot).  The pivot's row-1
+// element follows from the orthogonality of the rows (u0_unpack), to a few
+// ulps of the projected input.  FP32 u0 keeps both rows (six numbers): the
+// FP32 sweep is also at its VALU limit.  Tiles hold U0Planes<T>::n planes of
+// 32 sites; the allocation keeps six planes per field (it also stages FP32
+// input on upload).
+template <typename T>
+struct U0Planes {
+    static constexpr int n = kPlanes;
+};
+template <>
+struct U0Planes<double> {
+    static constexpr int n = 5;
+};
+
+template <typename T>
+__host__ __device__ constexpr int64_t u0_field_size(int64_t n) {
+    return (n + kTile - 1) / kTile * kTile * U0Planes<T>::n;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t u0_tile_bytes(int h) {
+    return ((uint32_t)(h >> 5) * (uint32_t)(kTile * U0Planes<T>::n) +
+            (uint32_t)(h & (kTile - 1))) * (uint32_t)sizeof(cplx<T>);
+}
+
 // Field of the spatial links u0_i on parity p; field of G on parity p.
 template <typename T>
 __device__ __forceinline__ const cplx<T> *u0_field(const cplx<T> *u0, int Vh, int p, int i) {
-    return u0 + (p * 3 + i) * field_size(Vh);
+    return u0 + (p * 3 + i) * u0_field_size<T>(Vh);
 }
 template <typename T>
 __device__ __forceinline__ cplx<T> *u0_field(cplx<T> *u0, int Vh, int p, int i) {
-    return u0 + (p * 3 + i) * field_size(Vh);
+    return u0 + (p * 3 + i) * u0_field_size<T>(Vh);
 }
 template <typename T>
 __device__ __forceinline__ const cplx<T> *g_field(const cplx<T> *g, int Vh, int p) {
@@ -122,6 +151,92 @@ __device__ __forceinline__ void load_rows01(mat3<T> &m, const cplx<T> *__restric
 #pragma unroll
     for (int k = 0; k < kPlanes; k++)
         m.e[k] = STREAM ? ld_stream(elem_ptr(field, off, k)) : *elem_ptr(field, off, k);
+}
+
+// Index of row 0's element of largest modulus (ties to the lower index); the
+// products are explicit FMAs, so a builder and a reader compute the same bits.
+__device__ __forceinline__ int u0_pivot(const cplx<double> *r0) {
+    const double n0 = __builtin_fma(r0[0].re, r0[0].re, r0[0].im * r0[0].im);
+    const double n1 = __builtin_fma(r0[1].re, r0[1].re, r0[1].im * r0[1].im);
+    const double n2 = __builtin_fma(r0[2].re, r0[2].re, r0[2].im * r0[2].im);
+    return (n0 >= n1 && n0 >= n2) ? 0 : (n1 >= n2 ? 1 : 2);
+}
+
+// Rows 0 and 1 of an FP64 u0 matrix from its five stored numbers e[0..4]:
+// row 0 = e[0..2]; row 1 = e[3], e[4] in the two non-pivot columns, and in the
+// pivot column k the element x with sum_j r0_j conj(r1_j) = 0, i.e.
+// x = -conj(S) r0_k / |r0_k|^2, S = sum_{j != k} r0_j conj(r1_j).  |r0_k|^2 is
+// at least 1/3, so the division is well conditioned.
+__device__ __forceinline__ cplx<double> csel(bool c, cplx<double> x, cplx<double> y) {
+    return c_make<double>(c ? x.re : y.re, c ? x.im : y.im);
+}
+
+__device__ __forceinline__ void u0_unpack(mat3<double> &m, const cplx<double> *e) {
+    const int k = u0_pivot(e);
+    const cplx<double> e0 = e[0], e1 = e[1], e2 = e[2], s0 = e[3], s1 = e[4];
+    const cplx<double> r = csel(k == 0, e0, csel(k == 1, e1, e2));
+    const cplx<double> a = csel(k == 0, e1, e0);  // row 0 over the first stored element
+    const cplx<double> b = csel(k == 2, e1, e2);  // ... and over the second
+    // S = a conj(s0) + b conj(s1)
+    const double sre = __builtin_fma(a.re, s0.re, __builtin_fma(a.im, s0.im,
+                       __builtin_fma(b.re, s1.re, b.im * s1.im)));
+    const double sim = __builtin_fma(a.im, s0.re, -__builtin_fma(a.re, s0.im,
+                       -__builtin_fma(b.im, s1.re, -(b.re * s1.im))));
+    const double nr = __builtin_fma(r.re, r.re, r.im * r.im);
+    const double inv = 1.0 / nr;
+    const cplx<double> x = c_make<double>(-__builtin_fma(sre, r.re, sim * r.im) * inv,
+                                          -__builtin_fma(sre, r.im, -(sim * r.re)) * inv);
+    m.e[0] = e0;
+    m.e[1] = e1;
+    m.e[2] = e2;
+    m.e[3] = csel(k == 0, x, s0);
+    m.e[4] = csel(k == 0, s0, csel(k == 1, x, s1));
+    m.e[5] = csel(k == 2, x, s1);
+}
+
+// The N stored numbers of a matrix at byte offset off of a field, as loaded.
+template <bool STREAM, typename T, int N>
+__device__ __forceinline__ void load_raw(cplx<T> (&e)[N], const cplx<T> *__restrict__ field,
+                                         uint32_t off) {
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        e[k] = STREAM ? ld_stream(elem_ptr(field, off, k)) : *elem_ptr(field, off, k);
+}
+
+// Rows 0 and 1 (row 2 left unset) from the stored numbers of a G matrix ...
+template <typename T>
+__device__ __forceinline__ void rows01(mat3<T> &m, const cplx<T> (&e)[kPlanes]) {
+#pragma unroll
+    for (int k = 0; k < kPlanes; k++) m.e[k] = e[k];
+}
+
+// ... and of a u0 matrix.
+template <typename T>
+__device__ __forceinline__ void u0_rows01(mat3<T> &m, const cplx<T> (&e)[U0Planes<T>::n]) {
+    if constexpr (U0Planes<T>::n == kPlanes)
+        rows01(m, e);
+    else
+        u0_unpack(m, e);
+}
+
+// Rows 0 and 1 of a u0 matrix (row 2 left unset).
+template <bool STREAM, typename T>
+__device__ __forceinline__ void load_u0_rows01(mat3<T> &m, const cplx<T> *__restrict__ field,
+                                               int h) {
+    cplx<T> e[U0Planes<T>::n];
+    load_raw<STREAM>(e, field, u0_tile_bytes<T>(h));
+    u0_rows01(m, e);
+}
+
+// A u0 matrix in FP64 (rows 0 and 1 as stored, row 2 rebuilt).
+template <typename T>
+__device__ __forceinline__ void load_u0_d(mat3<double> &m, const cplx<T> *__restrict__ field,
+                                          int h) {
+    mat3<T> t;
+    load_u0_rows01<false>(t, field, h);
+#pragma unroll
+    for (int k = 0; k < kPlanes; k++) m.e[k] = c_make<double>((double)t.e[k].re, (double)t.e[k].im);
+    m_fill_row2(m);
 }
 
 template <bool STREAM, typename T>
@@ -217,17 +332,29 @@ __device__ __forceinline__ void sweep_site(const cplx<T> *__restrict__ u0,
         __builtin_amdgcn_sched_barrier(0);
     }
     // the six products are SU(3): rows 0, 1 from the stored rows of the left
-    // factor, row 2 by closure (m_mac_nd_su3, m_mac_dagger_su3)
+    // factor, row 2 by closure (m_mac_nd_su3, m_mac_dagger_su3).  The four
+    // matrices of a direction are loaded together, one memory round trip per
+    // direction rather than the compiler's two (48^3 x 96: FP64 fix 1%, mixed
+    // 2% faster).
     mat3<T> W;
     m_zero(W);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
+        constexpr int NU = U0Planes<T>::n;
+        cplx<T> ruf[NU], rub[NU], rgf[kPlanes], rgb[kPlanes];
+        load_raw<STREAM>(ruf, u0_field(u0, Vh, p, i), u0_tile_bytes<T>(h));
+        load_raw<false>(rgf, gq, tile_bytes<T>(nb.fwd[i]));
+        load_raw<STREAM>(rub, u0_field(u0, Vh, 1 - p, i), u0_tile_bytes<T>(nb.bwd[i]));
+        load_raw<false>(rgb, gq, tile_bytes<T>(nb.bwd[i]));
+        __builtin_amdgcn_sched_barrier(0);
         mat3<T> uf, gf, ub, gb;
-        load_rows01<STREAM>(uf, u0_field(u0, Vh, p, i), h);
-        load_su3<false>(gf, gq, nb.fwd[i]);
+        u0_rows01(uf, ruf);
+        rows01(gf, rgf);
+        m_fill_row2(gf);
         m_mac_nd_su3(W, uf, gf);  // u0_i(n) G(n+i)^dagger
-        load_su3<STREAM>(ub, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
-        load_rows01<false>(gb, gq, nb.bwd[i]);
+        u0_rows01(ub, rub);
+        m_fill_row2(ub);
+        rows01(gb, rgb);
         m_mac_dagger_su3(W, gb, ub);  // (G(n-i) u0_i(n-i))^dagger
     }
     if (PIN)
@@ -344,16 +471,25 @@ __global__ __launch_bounds__(kBlock) void k_build_u0(const cplx<double> *__restr
     const int p = idx >= Vh, h = idx - p * Vh;
     const Site s = site_of(h, p, ns);
     const int64_t lex = (int64_t)2 * s.row + s.x;
-    const uint32_t off = tile_bytes<T>(h);
+    const uint32_t off = u0_tile_bytes<T>(h);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const cplx<double> *src = cfg + (lex * 4 + 1 + i) * 9;
         cplx<T> *dst = u0_field(u0, Vh, p, i);
+        cplx<double> e[kPlanes];
 #pragma unroll
-        for (int k = 0; k < kPlanes; k++) {
-            cplx<double> v = src[k];
-            *elem_ptr(dst, off, k) = c_make<T>((T)v.re, (T)v.im);
+        for (int k = 0; k < kPlanes; k++) e[k] = src[k];
+        if constexpr (U0Planes<T>::n == 5) {
+            // row 1 without the element under row 0's pivot (u0_unpack)
+            const int piv = u0_pivot(e);
+            const cplx<double> s0 = csel(piv == 0, e[4], e[3]);
+            const cplx<double> s1 = csel(piv == 2, e[4], e[5]);
+            e[3] = s0;
+            e[4] = s1;
         }
+#pragma unroll
+        for (int k = 0; k < U0Planes<T>::n; k++)
+            *elem_ptr(dst, off, k) = c_make<T>((T)e[k].re, (T)e[k].im);
     }
 }
 
@@ -544,13 +680,13 @@ __device__ __forceinline__ Obs obs_site(const cplx<TU> *__restrict__ u0,
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             mat3<double> u, gm, tmp, lk;
-            load_su3_d(u, u0_field(u0, Vh, p, i), h);
+            load_u0_d(u, u0_field(u0, Vh, p, i), h);
             load_su3_d(gm, gq, nb.fwd[i]);
             m_mul(gn, u, tmp);
             m_mul_nd(tmp, gm, lk);  // U_i(n)
             accum_link(D, lk, 1.0);
             f += lk.e[0].re + lk.e[4].re + lk.e[8].re;
-            load_su3_d(u, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
+            load_u0_d(u, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
             load_su3_d(gm, gq, nb.bwd[i]);
             m_mul(gm, u, tmp);
             m_mul_nd(tmp, gn, lk);  // U_i(n-i)
