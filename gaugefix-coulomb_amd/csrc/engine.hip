@@ -488,7 +488,9 @@ struct SweepState {
     bool fp32 = false;
 };
 
-// Bytes the sweeps read of one timeslice: u0 (3 links) and G, two rows each.
+// Bytes of one timeslice as the cache-budget policy counts them: u0 (3 links)
+// and G at six complex numbers each (an FP64 u0 link reads five; the policy
+// below was measured with this count).
 int64_t slice_bytes(const gfc_job *j, bool fp32) {
     const int64_t S = (int64_t)j->geom.ns * j->geom.ns * j->geom.ns;
     return S * 4 * 6 * (fp32 ? 8 : 16);

@@ -72,7 +72,7 @@ struct gfc_job {
     int Vh = 0;
     int precision = GFC_PREC_FP64;
     gfc::cplx<double> *cfg = nullptr;    // AoS FP64 U, 4V matrices
-    gfc::cplx<double> *u0 = nullptr;     // SoA spatial links, FP64 (two rows)
+    gfc::cplx<double> *u0 = nullptr;     // SoA spatial links, FP64 (five numbers each)
     gfc::cplx<double> *g = nullptr;      // SoA G, FP64 (two rows)
     gfc::cplx<float> *u0f = nullptr;     // SoA spatial links, FP32 (MIXED)
     gfc::cplx<float> *gf = nullptr;      // SoA G, FP32 (MIXED)

@@ -4,8 +4,9 @@
 //   cfg : AoS FP64, the reference's U image (4V matrices, site-major, mu=T,Z,Y,X)
 //   u0  : SoA spatial links of the configuration being fixed
 //   g   : SoA gauge transformation
-// u0 and g hold SU(3) matrices as their first two rows (6 complex numbers);
-// row 2 = conj(r0 x r1) is rebuilt in registers on every load.  One tiled field
+// u0 and g hold SU(3) matrices as their first two rows (6 complex numbers; an
+// FP64 u0 link as five, see Stored below); row 2 = conj(r0 x r1) is rebuilt in
+// registers on every load.  One tiled field
 // (see field_size below) per parity p and, for u0, direction i = Z,Y,X:
 //   u0 field p*3 + i,   g field p.
 // Within a parity the index is h = lex >> 1 with lex = ((t ns + z) ns + y) ns + x
@@ -25,7 +26,7 @@
 namespace gfc {
 
 constexpr int kBlock = 256;
-constexpr int kPlanes = 6;  // stored complex elements per SU(3) matrix
+constexpr int kPlanes = 6;  // planes of a tile: complex numbers of a two-row matrix
 
 // Kernels and device helpers have internal linkage: each translation unit
 // that launches them (engine.hip, server.hip) carries its own code object.
