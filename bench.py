@@ -13,9 +13,9 @@ value        site updates per second of the FP64 drop-in path
 fp64_seconds wall time to e2 < 1e-16, FP64 path
 mixed_*      the FP32 -> FP64 path on the same input and schedule
 roofline     the kernel of the timed fix, k_sweep<double> in cache-resident
-             timeslice waves: the factored update's algorithmic bytes (864 B
-             per site update: two stored rows of the 6 spatial u0 links, G(n)
-             read and written, one neighbour G per update) per lattice
+             timeslice waves: the factored update's algorithmic bytes (768 B
+             per site update: the 6 spatial u0 links in five stored complex
+             numbers, G(n) read and written, one neighbour G per update) per lattice
              half-sweep over the half-sweep's duration measured with HIP
              events on the job's stream (fork to join of the waves), and the
              copy-calibrated L2-miss traffic of the same launches from the
@@ -38,6 +38,14 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gaugefix-coulomb_amd"))
+
+# The serving leg's server runs seven streams (one per size lane, three more
+# for the large lane's timeslice waves).  With HIP's default of four hardware
+# queues some of them share a queue, and kernels on one queue run in order: a
+# lone 48^3x96 request then takes 0.92 s instead of 0.72 s.  Eight queues
+# (read when the HIP runtime starts, so set before any HIP call) give each
+# stream its own; the single-lattice fixes use four streams either way.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 # bytes per site update credited to the reference's update: 6 spatial links
 # and G(n), full 3x3 matrices read and written once (14 x 144 B); only used
@@ -538,6 +546,9 @@ def main():
         serve_out["lone_service_ms"] = {f"{k[0]}^3x{k[1]} tol {k[2]:.0e}": round(v[0] * 1e3, 2)
                                         for k, v in sorted(lone.items())}
         serve_out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in serve_out.items()}
+        serve_out["by_size"] = {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv)
+                                    for kk, vv in v.items()}
+                                for k, v in serve_out["by_size"].items()}
         serve_out["workload"] = ("config 4: sizes {16^3x32: 40%, 24^3x48: 30%, 32^3x64: 20%, "
                                  "48^3x96: 10%}, eps U[0.2,0.4], tol {1e-14, 1e-16}, "
                                  f"Poisson {a.serve_rate}/s, inputs synthesised in HBM")

@@ -76,6 +76,13 @@ def run_stream(server: Server, reqs, options: Options | None = None, log=None):
                 f"pending={server.pending()}")
     comps = [by_tag[k] for k in range(n)]
     lat = np.array([c.t_done - (t0 + reqs[c.tag].t_arrival) for c in comps])
+    # latency by lattice size
+    by_size = {}
+    for key in sorted({(c.ns, c.nt) for c in comps}):
+        l = np.array([lat[i] for i, c in enumerate(comps) if (c.ns, c.nt) == key])
+        by_size[f"{key[0]}^3x{key[1]}"] = {"requests": int(l.size),
+                                          "p50_ms": float(np.percentile(l, 50) * 1e3),
+                                          "p99_ms": float(np.percentile(l, 99) * 1e3)}
     makespan = max(c.t_done for c in comps) - t0
     upd = sum(c.sweeps * (c.ns ** 3 * c.nt // 2) for c in comps)
     st = server.stats()
@@ -93,6 +100,7 @@ def run_stream(server: Server, reqs, options: Options | None = None, log=None):
                              for c in comps),
         "chunks": st["chunks"], "graph_replays": st["graph_replays"],
         "graphs_built": st["graphs_built"],
+        "by_size": by_size,
     }, comps
 
 

@@ -329,7 +329,9 @@ typedef struct {
  * size (fewer than 2^18, 2^20, 2^22 sites, and larger), one stream each, so
  * that small lattices do not advance at the pace of large ones; the large
  * lane sweeps each lattice in the cache-resident timeslice waves of
- * gfc_job_fix, on three more streams of its own.  mem_budget
+ * gfc_job_fix, on three more streams of its own.  Seven streams need eight
+ * hardware queues (GPU_MAX_HW_QUEUES=8 before the HIP runtime starts): with
+ * HIP's default of four, some share a queue and run one after the other.  mem_budget
  * (bytes, <= 0: 85% of the free HBM) bounds the summed footprints of resident
  * lattices; max_jobs bounds their number; keep_results keeps finished
  * lattices resident until released.  gfc_server_poll never waits for a busy
