@@ -275,7 +275,10 @@ int gfc_job_write_config(gfc_job *job, const char *path, int out_precision, int 
 /* The drop-in entry point: gaugefixOverrelaxation(U, G, params) on host
  * arrays (include/gauge_fixing.h:67-69).  U (4*V*18 doubles) and G (V*18
  * doubles) are updated in place with the same return codes.  o and rep may
- * be NULL. */
+ * be NULL.  U must hold SU(3) links, as the reference's main leaves them
+ * (reunitarizeField before the call, gauge_fix_coulomb.c:126): the engine
+ * keeps each link as its first two rows (an FP64 link as row 0 and two
+ * elements of row 1) and rebuilds the rest from unitarity. */
 int gfc_fix_coulomb_or(gfc_ctx *ctx, const gfc_geom *g, const gfc_params *p,
                        const gfc_options *o, double *U, double *G, gfc_report *rep);
 /* reunitarizeField (src/fields.c:144-187) on a host array of n matrices. */
