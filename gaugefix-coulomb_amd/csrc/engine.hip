@@ -595,13 +595,15 @@ int reunit_g(gfc_job *j, hipStream_t s, bool fp32, int p0, int np) {
 // Timeslices per group, or 0 for whole-lattice sweeps, for the sweeps in
 // precision fp32.  The groups are spread over the context's streams, a wave
 // of one group per stream at a time, so that one group's kernel tail overlaps
-// the others' work.  By default (slice_group = 0):
-// * a lattice whose u0 and G fit in kCacheBytes is swept whole, from the cache;
-// * otherwise, if a wave of single timeslices fits, the largest groups whose
-//   wave fits, evened out so that every wave fills all streams;
+// the others' work.  By default (slice_group = 0), with L = lane_count:
+// * a lattice whose u0 and G fit in kCacheBytes: nt / L timeslices per group,
+//   or the whole lattice in one launch when L = 1; from the cache;
+// * otherwise, if a wave of L single timeslices fits, the largest groups whose
+//   wave fits, evened out so that every wave fills all streams (64^3 x 128 in
+//   FP64: single timeslices on two streams, from the cache);
 // * otherwise single timeslices with non-temporal loads (a timeslice is then
-//   at least 48 MiB, ns >= 51 in FP64, so a launch still fills the GPU; at
-//   64^3 x 128 in FP64 this is 8-10% faster than groups of 8 timeslices).
+//   large enough that a launch still fills the GPU; single timeslices on four
+//   streams were 8-10% faster than groups of 8 timeslices at 64^3 x 128).
 int group_size(const gfc_job *j, const gfc_options &o, bool fp32) {
     if (o.slice_group < 0) return 0;
     const int nt = j->geom.nt;
