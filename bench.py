@@ -262,8 +262,9 @@ def io_leg(job, geom, U):
 
 def config5(ctx, ns: int, nt: int, eps: float, seed: int):
     """64^3x128 (33.5 M sites): FP64, mixed, and mixed with per-timeslice
-    retirement on the reference schedule, plus the per-launch GB/s of the FP64
-    and FP32 sweep kernels (whole-lattice launches, HIP events).  The input is
+    retirement on the reference schedule, plus the GB/s of the FP64 and FP32
+    sweep kernels: as the fixes run them (half-sweeps in the default grouping)
+    and in whole-lattice launches (HIP events).  The input is
     synthesised in HBM by the device generator (the host generator's
     construction), so no 19 GB host image is needed."""
     import gfc
@@ -302,6 +303,27 @@ def config5(ctx, ns: int, nt: int, eps: float, seed: int):
         out["mixed_speedup_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
         out["mixed_retire_speedup_vs_fp64"] = round(
             out["fp64"]["seconds"] / out["mixed_retire"]["seconds"], 3)
+        # the fixes' own sweeps (default grouping, folded reunitarization): HIP
+        # events around the runs of blocked sweeps, 200 half-sweeps; traffic from
+        # the committed counters of the same sweeps
+        fixed200 = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
+                              max_sweeps_to_fix=200, sweeps_to_reunitarization=10)
+        for name, opt, bpu, prof in (
+                ("fp64", gfc.Options(precision=gfc.PREC_FP64, time_sweeps=1),
+                 ALG_BYTES_PER_UPDATE_FP64, f"sweep_fix_{ns}x{nt}_pmc.txt"),
+                ("fp32", gfc.Options(precision=gfc.PREC_MIXED, mixed_switch=0.0,
+                                     mixed_probe_every=10 ** 6, time_sweeps=1),
+                 ALG_BYTES_PER_UPDATE_FP32, f"sweep_fix_fp32_{ns}x{nt}_pmc.txt")):
+            job.restore_config()
+            rc, rep = job.fix(fixed200, opt)
+            ms = rep.seconds_sweeps / rep.sweeps * 1e3
+            t_upd = pmc_value(prof, f"fix_{name}_traffic_bytes_per_update")
+            out[f"fix_{name}_half_sweep_ms"] = round(ms, 4)
+            out[f"fix_{name}_GBs_algorithmic"] = round(bpu * vh / (ms * 1e-3) / 1e9, 1)
+            out[f"fix_{name}_traffic_bytes_per_update"] = t_upd
+            if t_upd:
+                out[f"fix_{name}_traffic_GBs"] = round(t_upd * vh / (ms * 1e-3) / 1e9, 1)
+            log(f"config5 {name} fix sweeps: {ms:.4f} ms per half-sweep")
         fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
                            max_sweeps_to_fix=40, sweeps_to_reunitarization=10 ** 6)
         for name, opt, bpu in (("fp64", gfc.PREC_FP64, ALG_BYTES_PER_UPDATE_FP64),
@@ -369,6 +391,7 @@ def main():
             job.generate_smooth(a.seed, a.eps)
             job.reunitarize()
             job.save_config()
+            log(f"profile: {a.ns}^3x{a.nt} input synthesised in HBM")
             profile_fix(ctx, job, a.profile_fix, a.profile_fp32)
         ctx.close()
         return
