@@ -75,8 +75,15 @@ def run_stream(server: Server, reqs, options: Options | None = None, log=None):
             log(f"serve: t={now:6.1f}s submitted={i} done={len(by_tag)} "
                 f"pending={server.pending()}")
     comps = [by_tag[k] for k in range(n)]
+    return summarise(reqs, comps, t0, server.stats()), comps
+
+
+def summarise(reqs, comps, t0: float, stats: dict) -> dict:
+    """Latency summary of a served stream: comps[k] is the completion of reqs[k]
+    (times on the server clock, the stream started at t0).  Latency is the
+    completion time minus the scheduled arrival; by_size splits it by lattice."""
+    n = len(reqs)
     lat = np.array([c.t_done - (t0 + reqs[c.tag].t_arrival) for c in comps])
-    # latency by lattice size
     by_size = {}
     for key in sorted({(c.ns, c.nt) for c in comps}):
         l = np.array([lat[i] for i, c in enumerate(comps) if (c.ns, c.nt) == key])
@@ -85,7 +92,6 @@ def run_stream(server: Server, reqs, options: Options | None = None, log=None):
                                           "p99_ms": float(np.percentile(l, 99) * 1e3)}
     makespan = max(c.t_done for c in comps) - t0
     upd = sum(c.sweeps * (c.ns ** 3 * c.nt // 2) for c in comps)
-    st = server.stats()
     return {
         "requests": n,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
@@ -98,10 +104,10 @@ def run_stream(server: Server, reqs, options: Options | None = None, log=None):
         "site_updates_per_s": upd / makespan,
         "all_converged": all(c.status >= 0 and c.residue_final <= reqs[c.tag].tolerance
                              for c in comps),
-        "chunks": st["chunks"], "graph_replays": st["graph_replays"],
-        "graphs_built": st["graphs_built"],
+        "chunks": stats["chunks"], "graph_replays": stats["graph_replays"],
+        "graphs_built": stats["graphs_built"],
         "by_size": by_size,
-    }, comps
+    }
 
 
 def lone_service_times(ctx, reqs, options: Options | None = None, reps: int = 2):
