@@ -94,7 +94,9 @@ __device__ __forceinline__ uint32_t tile_bytes(int h) {
 // sit under row 0's element of largest modulus (its pivot).  The pivot's row-1
 // element follows from the orthogonality of the rows (u0_unpack), to a few
 // ulps of the projected input.  FP32 u0 keeps both rows (six numbers): the
-// FP32 sweep is also at its VALU limit.  Tiles hold U0Planes<T>::n planes of
+// FP32 sweep is also at its VALU limit.  G keeps both rows too: stored in five
+// numbers it made the 48^3 x 96 FP64 fix 6% slower (the unpacking sits on the
+// neighbour loads' path).  Tiles hold U0Planes<T>::n planes of
 // 32 sites; the allocation keeps six planes per field (it also stages FP32
 // input on upload).
 template <typename T>
