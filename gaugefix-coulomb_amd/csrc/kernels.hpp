@@ -805,24 +805,30 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const double2 *__restrict__ i
     if (i < n) out[i] = in[i];
 }
 
-// Reads n 16-byte elements, grid-stride with eight loads in flight per lane,
-// and stores only if their sum takes a value a zeroed buffer cannot give (a
-// read-bandwidth probe: read repeatedly, a buffer of at most 256 MiB stays in
-// the Infinity Cache).
-__global__ __launch_bounds__(kBlock) void k_read16(const double2 *__restrict__ in, int64_t n,
-                                                   double2 *__restrict__ sink) {
+// Read-rate probe: `passes` reads of gridDim.x contiguous chunks of `chunk`
+// 16-byte elements, eight loads in flight per lane.  In pass p workgroup b
+// reads chunk (b + p) mod gridDim.x, so no workgroup reads the same bytes twice
+// in a row and the next reader of a chunk runs on another XCD: every pass
+// streams the buffer from where it resides (the Infinity Cache up to 256 MiB,
+// HBM above), not from an L2.  The sum is stored only if it takes a value a
+// zeroed buffer cannot give.
+__global__ __launch_bounds__(kBlock) void k_read16(const double2 *__restrict__ in, int64_t chunk,
+                                                   int passes, double2 *__restrict__ sink) {
     constexpr int kInFlight = 8;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double2 acc = {0.0, 0.0};
-    for (; i + (kInFlight - 1) * stride < n; i += kInFlight * stride) {
-        double2 v[kInFlight];
+    for (int p = 0; p < passes; p++) {
+        asm volatile("" ::: "memory");  // every pass loads again
+        const double2 *c = in + (int64_t)((blockIdx.x + (unsigned)p) % gridDim.x) * chunk;
+        int64_t j = threadIdx.x;
+        for (; j + (kInFlight - 1) * kBlock < chunk; j += kInFlight * kBlock) {
+            double2 v[kInFlight];
 #pragma unroll
-        for (int u = 0; u < kInFlight; u++) v[u] = in[i + u * stride];
+            for (int u = 0; u < kInFlight; u++) v[u] = c[j + u * kBlock];
 #pragma unroll
-        for (int u = 0; u < kInFlight; u++) acc += v[u];
+            for (int u = 0; u < kInFlight; u++) acc += v[u];
+        }
+        for (; j < chunk; j += kBlock) acc += c[j];
     }
-    for (; i < n; i += stride) acc += in[i];
     if (acc.x == 1.2345e-300) sink[(int64_t)blockIdx.x * kBlock + threadIdx.x] = acc;
 }
 

@@ -370,8 +370,9 @@ class Context:
         return s.value
 
     def read_probe(self, nbytes: int, reps: int = 20) -> float:
-        """Mean seconds of one read pass over a device buffer of nbytes
-        (Infinity-Cache resident up to 256 MiB, HBM above)."""
+        """Seconds of one read pass over a device buffer of nbytes (Infinity-Cache
+        resident up to 256 MiB, HBM above), without the launch's ramp-up and
+        drain: many passes per launch, differenced over reps passes."""
         s = C.c_double()
         _check(lib.gfc_ctx_read_probe(self._p, nbytes, reps, C.byref(s)))
         return s.value

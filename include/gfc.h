@@ -202,11 +202,15 @@ void gfc_ctx_destroy(gfc_ctx *ctx);
  * bandwidth and calibrates memory-traffic counters (no counterpart in the
  * reference). */
 int gfc_ctx_copy_probe(gfc_ctx *ctx, int64_t bytes, int reps, double *seconds);
-/* Reads a zeroed buffer of `bytes` bytes `reps` times (after three untimed
- * passes) with a 16-byte-per-lane read kernel; *seconds receives the mean
- * time per pass from HIP events.  A buffer of at most 256 MiB is then read
- * from the Infinity Cache, a larger one from HBM: the read bandwidth that
- * bounds the sweeps' L2-miss traffic (no counterpart in the reference). */
+/* Streams a zeroed device buffer of `bytes` bytes (rounded down to a whole
+ * number of 16-byte elements per workgroup chunk: exact for multiples of
+ * 64 KiB) through a 16-byte-per-lane read kernel, many passes in one launch
+ * with the chunks rotating over the workgroups so that no pass is served by
+ * an L2.  *seconds receives the time of one pass without the launch's ramp-up
+ * and drain: the difference of a (2 + reps)-pass and a 2-pass launch over
+ * reps, from HIP events, best of three.  A buffer of at most 256 MiB is read
+ * from the Infinity Cache, a larger one from HBM: the read rates that bound
+ * the sweeps' L2-miss traffic (no counterpart in the reference). */
 int gfc_ctx_read_probe(gfc_ctx *ctx, int64_t bytes, int reps, double *seconds);
 /* Bytes of device memory a job of this geometry needs for the given
  * precision (used for admission by footprint). */
