@@ -484,12 +484,13 @@ int prepare_fix(gfc_job *j, bool mixed) {
 int materialize(gfc_job *j, bool project) { return materialize(j, j->ctx->stream, project); }
 
 int materialize(gfc_job *j, hipStream_t s, bool project) {
+    // one thread per link
     if (project)
-        hipLaunchKernelGGL(gfc::k_materialize<true>, dim3(nblocks(j->V)), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(gfc::k_materialize<true>, dim3(nblocks(4 * j->V)), dim3(kBlock), 0, s,
                            j->cfg, j->g, j->geom.ns, j->geom.nt, j->Vh);
     else
-        hipLaunchKernelGGL(gfc::k_materialize<false>, dim3(nblocks(j->V)), dim3(kBlock), 0, s,
-                           j->cfg, j->g, j->geom.ns, j->geom.nt, j->Vh);
+        hipLaunchKernelGGL(gfc::k_materialize<false>, dim3(nblocks(4 * j->V)), dim3(kBlock), 0,
+                           s, j->cfg, j->g, j->geom.ns, j->geom.nt, j->Vh);
     LAUNCH_CHECK();
     return GFC_OK;
 }

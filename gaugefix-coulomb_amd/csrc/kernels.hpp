@@ -499,40 +499,54 @@ __global__ __launch_bounds__(kBlock) void k_build_u0(const cplx<double> *__restr
 // cfg <- G(n) cfg G(n+mu)^dagger for all four directions
 // (applyGaugeTransformationU, src/fields.c:375-435); with PROJECT every link
 // is then projected onto SU(3), the reunitarization of U that ends a
-// converged gaugefixOverrelaxation (src/gauge_fixing.c:822).
+// converged gaugefixOverrelaxation (src/gauge_fixing.c:822).  One thread per
+// link, 64 sites (256 links, 36 KiB) per workgroup: the links are read from
+// and written back to the AoS array through LDS in consecutive 16-byte
+// pieces, so that every wave's loads and stores cover whole cache lines (one
+// thread per site reading its 576 bytes directly left the accesses of a wave
+// 576 bytes apart).
 template <bool PROJECT>
 __global__ __launch_bounds__(kBlock) void k_materialize(cplx<double> *__restrict__ cfg,
                                                         const cplx<double> *__restrict__ g,
                                                         int ns, int nt, int Vh) {
-    const int64_t lex = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (lex >= 2 * (int64_t)Vh) return;
-    const int x = (int)(lex % ns);
-    int64_t r = lex / ns;
-    const int y = (int)(r % ns);
-    r /= ns;
-    const int z = (int)(r % ns);
-    const int t = (int)(r / ns);
-    const int p = (x + y + z + t) & 1;
-    mat3<double> gn;
-    load_su3<false>(gn, g_field(g, Vh, p), (int)(lex >> 1));
-    const cplx<double> *gq = g_field(g, Vh, 1 - p);
-#pragma unroll
-    for (int mu = 0; mu < 4; mu++) {
+    __shared__ double2 sh[kBlock * 9];
+    const int64_t link0 = (int64_t)blockIdx.x * kBlock;
+    const int nl = (int)min<int64_t>(kBlock, 8 * (int64_t)Vh - link0);  // links here
+    double2 *base = reinterpret_cast<double2 *>(cfg) + link0 * 9;
+    for (int k = threadIdx.x; k < nl * 9; k += kBlock) sh[k] = base[k];
+    __syncthreads();
+    if ((int)threadIdx.x < nl) {
+        const int64_t link = link0 + threadIdx.x;
+        const int64_t lex = link >> 2;
+        const int mu = (int)(link & 3);
+        const int x = (int)(lex % ns);
+        int64_t r = lex / ns;
+        const int y = (int)(r % ns);
+        r /= ns;
+        const int z = (int)(r % ns);
+        const int t = (int)(r / ns);
+        const int p = (x + y + z + t) & 1;
         int c[4] = {t, z, y, x};
         c[mu] += 1;
         if (c[mu] == (mu == 0 ? nt : ns)) c[mu] = 0;
         const int64_t ln = (((int64_t)c[0] * ns + c[1]) * ns + c[2]) * ns + c[3];
-        mat3<double> gm, u, tmp, out;
-        load_su3<false>(gm, gq, (int)(ln >> 1));
-        cplx<double> *ptr = cfg + (lex * 4 + mu) * 9;
+        mat3<double> gn, gm, u, tmp, out;
+        load_su3<false>(gn, g_field(g, Vh, p), (int)(lex >> 1));
+        load_su3<false>(gm, g_field(g, Vh, 1 - p), (int)(ln >> 1));
+        double2 *mine = sh + threadIdx.x * 9;
 #pragma unroll
-        for (int k = 0; k < 9; k++) u.e[k] = ptr[k];
+        for (int k = 0; k < 9; k++) {
+            u.e[k].re = mine[k].x;
+            u.e[k].im = mine[k].y;
+        }
         m_mul(gn, u, tmp);
         m_mul_nd(tmp, gm, out);
         if (PROJECT) m_project_su3(out);
 #pragma unroll
-        for (int k = 0; k < 9; k++) ptr[k] = out.e[k];
+        for (int k = 0; k < 9; k++) mine[k] = double2{out.e[k].re, out.e[k].im};
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nl * 9; k += kBlock) base[k] = sh[k];
 }
 
 // Projects n AoS FP64 matrices onto SU(3); sets *flag when a row was null
