@@ -175,7 +175,8 @@ int gfc_ctx_copy_probe(gfc_ctx *c, int64_t bytes, int reps, double *seconds) {
 }
 
 int gfc_ctx_read_probe(gfc_ctx *c, int64_t bytes, int reps, double *seconds) {
-    if (c == nullptr || bytes < 16 || reps < 1 || seconds == nullptr)
+    // reps + 2 passes run in one launch
+    if (c == nullptr || bytes < 16 || reps < 1 || reps > (1 << 20) || seconds == nullptr)
         return fail(GFC_EINVAL, "gfc_ctx_read_probe: bad argument");
     HIP_TRY(hipSetDevice(c->device));
     const int64_t n = bytes / 16;

@@ -208,7 +208,7 @@ int gfc_ctx_copy_probe(gfc_ctx *ctx, int64_t bytes, int reps, double *seconds);
  * with the chunks rotating over the workgroups so that no pass is served by
  * an L2.  *seconds receives the time of one pass without the launch's ramp-up
  * and drain: the difference of a (2 + reps)-pass and a 2-pass launch over
- * reps, from HIP events, best of three.  A buffer of at most 256 MiB is read
+ * reps (1 to 2^20), from HIP events, best of three.  A buffer of at most 256 MiB is read
  * from the Infinity Cache, a larger one from HBM: the read rates that bound
  * the sweeps' L2-miss traffic (no counterpart in the reference). */
 int gfc_ctx_read_probe(gfc_ctx *ctx, int64_t bytes, int reps, double *seconds);

@@ -20,3 +20,5 @@ def test_copy_and_read_probes(ctx):
         ctx.read_probe(8, 1)
     with pytest.raises(gfc.GfcError):
         ctx.copy_probe(1 << 20, 0)
+    with pytest.raises(gfc.GfcError):
+        ctx.read_probe(1 << 20, (1 << 20) + 1)
