@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -361,6 +362,9 @@ class Context:
 
     def __init__(self, device: int = 0):
         self._p = C.c_void_p()
+        # jobs and servers on this context: closed before it (their C objects
+        # use the context's streams when they are destroyed)
+        self._children = weakref.WeakSet()
         _check(lib.gfc_ctx_create(device, C.byref(self._p)))
 
     def copy_probe(self, nbytes: int, reps: int = 5) -> float:
@@ -379,6 +383,8 @@ class Context:
 
     def close(self):
         if self._p:
+            for child in list(self._children):
+                child.close()
             lib.gfc_ctx_destroy(self._p)
             self._p = C.c_void_p()
 
@@ -401,6 +407,7 @@ class Job:
         self._p = C.c_void_p()
         g = geom._c()
         _check(lib.gfc_job_create(ctx._p, C.byref(g), precision, C.byref(self._p)))
+        ctx._children.add(self)
 
     def close(self):
         if self._p:
@@ -575,6 +582,7 @@ class Server:
         self._p = C.c_void_p()
         _check(lib.gfc_server_create(ctx._p, mem_budget, max_jobs, int(keep_results),
                                      C.byref(self._p)))
+        ctx._children.add(self)
         self._buf = (_Completion * 64)()
 
     def close(self):

@@ -195,6 +195,7 @@ const char *gfc_last_error(void);
 const char *gfc_version(void);
 
 int gfc_ctx_create(int device, gfc_ctx **ctx);
+/* Destroy a context's jobs and servers first: they use its streams. */
 void gfc_ctx_destroy(gfc_ctx *ctx);
 /* Device copy of `bytes` bytes (a plain 16-byte-per-lane streaming copy
  * kernel) repeated `reps` times on the context's stream; *seconds receives
