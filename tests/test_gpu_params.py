@@ -102,3 +102,28 @@ def test_run_to_tolerance(ctx, ns, nt, seed, eps, omega, hits, R, est):
     assert e2 <= tol
     assert dG < G_ABS
     job.close()
+
+
+@pytest.mark.parametrize("omega,hits,R,est", VARIANTS)
+def test_mixed_precision_variants(ctx, omega, hits, R, est):
+    """The FP32 -> FP64 path under the same settings, on 8^3x16: it stops at a
+    scheduled check no earlier than the FP64 run's, with e2 at or below the
+    tolerance and the gauge functional F equal to the FP64 one to 1e-10
+    relative (the same gauge copy)."""
+    geom = gfc.Geometry(8, 16)
+    p = gfc.Params(tolerance=1e-14, estimate_sweeps_to_gf_progress=est,
+                   sweeps_to_reunitarization=R, omega_OR=omega, hits=hits)
+    with gfc.Job(ctx, geom, gfc.PREC_MIXED) as job:
+        job.generate_smooth(21, 0.3)
+        job.reunitarize()
+        job.save_config()
+        rc64, _ = job.fix(p, gfc.Options(precision=gfc.PREC_FP64))
+        F64 = job.observables()[2]
+        job.restore_config()
+        rc, rep = job.fix(p, gfc.Options(precision=gfc.PREC_MIXED))
+        e2, _, F = job.observables()
+        print(f"omega {omega} hits {hits} R {R} est {est}: FP64 {rc64}, mixed {rc} "
+              f"(fp32 {rep.sweeps_fp32}), e2 {e2:.2e}, F rel {abs(F - F64) / F64:.2e}")
+        assert rc64 > 0 and rc >= rc64
+        assert e2 <= 1e-14
+        assert abs(F - F64) / abs(F64) < 1e-10
