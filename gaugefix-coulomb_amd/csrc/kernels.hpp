@@ -678,7 +678,14 @@ __device__ __forceinline__ void block_reduce3(Obs o, double *__restrict__ part, 
 }
 
 // Observables of the factored state at site index idx in [0, 2 Vh) (parity
-// idx >= Vh); u0 of type TU, G of type TG.
+// idx >= Vh); u0 of type TU, G of type TG, arithmetic in FP64.  They come
+// from the site's w = G(n) W(n), formed as the sweep forms it:
+// w = sum_i [U_i(n) + U_i(n-i)^dagger], so w - w^dagger is the divergence
+// sum_i [(U - U^dagger)_i(n) - (U - U^dagger)_i(n-i)] that e2 and theta
+// measure (calculateA, divergenceA: src/four_potential.c), and
+// Re Tr w / 2 summed over all sites is the sum over links of Re Tr U that F
+// measures (calculateF, src/gauge_fixing.c).  Seven products instead of the
+// twelve of forming every link.
 template <typename TU, typename TG>
 __device__ __forceinline__ Obs obs_site(const cplx<TU> *__restrict__ u0,
                                         const cplx<TG> *__restrict__ g, int ns, int Vh,
@@ -689,39 +696,43 @@ __device__ __forceinline__ Obs obs_site(const cplx<TU> *__restrict__ u0,
         const Site s = site_of(h, p, ns);
         const Nbr nb = neighbours(s, h, ns);
         const cplx<TG> *gq = g_field(g, Vh, 1 - p);
-        mat3<double> gn;
-        load_su3_d(gn, g_field(g, Vh, p), h);
-        mat3<double> D;
-        m_zero(D);
-        double f = 0;
+        mat3<double> W;
+        m_zero(W);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            mat3<double> u, gm, tmp, lk;
-            load_u0_d(u, u0_field(u0, Vh, p, i), h);
-            load_su3_d(gm, gq, nb.fwd[i]);
-            m_mul(gn, u, tmp);
-            m_mul_nd(tmp, gm, lk);  // U_i(n)
-            accum_link(D, lk, 1.0);
-            f += lk.e[0].re + lk.e[4].re + lk.e[8].re;
-            load_u0_d(u, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
-            load_su3_d(gm, gq, nb.bwd[i]);
-            m_mul(gm, u, tmp);
-            m_mul_nd(tmp, gn, lk);  // U_i(n-i)
-            accum_link(D, lk, -1.0);
+            mat3<double> uf, gf, ub, gb;
+            load_u0_d(uf, u0_field(u0, Vh, p, i), h);
+            load_su3_d(gf, gq, nb.fwd[i]);
+            load_u0_d(ub, u0_field(u0, Vh, 1 - p, i), nb.bwd[i]);
+            load_su3_d(gb, gq, nb.bwd[i]);
+            m_mac_nd_su3(W, uf, gf);      // u0_i(n) G(n+i)^dagger
+            m_mac_dagger_su3(W, gb, ub);  // (G(n-i) u0_i(n-i))^dagger
+            // one direction's loads at a time (hoisting all of them took 2
+            // waves/SIMD)
+            __builtin_amdgcn_sched_barrier(0);
         }
-        o = site_obs(D, f);
+        mat3<double> gn, w, D;
+        load_su3_d(gn, g_field(g, Vh, p), h);
+        m_mul(gn, W, w);
+        m_zero(D);
+        accum_link(D, w, 1.0);
+        o = site_obs(D, 0.5 * (w.e[0].re + w.e[4].re + w.e[8].re));
     }
     return o;
 }
 
-// One thread per site, both parities; per-block partial sums in part.
+// One thread per site, both parities; per-block partial sums in part, in
+// site order.  The workgroups run in XCD-contiguous order (xcd_order), so that
+// the neighbour G and backward u0 links a site reads were mostly brought into
+// its XCD's L2 by the sites next to it.
 template <typename TU, typename TG>
 __global__ __launch_bounds__(kBlock) void k_obs_factored(const cplx<TU> *__restrict__ u0,
                                                          const cplx<TG> *__restrict__ g,
                                                          int ns, int Vh,
                                                          double *__restrict__ part) {
-    const int idx = blockIdx.x * kBlock + threadIdx.x;
-    block_reduce3(obs_site<TU, TG>(u0, g, ns, Vh, idx), part, blockIdx.x);
+    const unsigned b = xcd_order(blockIdx.x, gridDim.x);
+    const int idx = (int)(b * kBlock + threadIdx.x);
+    block_reduce3(obs_site<TU, TG>(u0, g, ns, Vh, idx), part, b);
 }
 
 // Observables of an AoS FP64 configuration; one thread per site.
