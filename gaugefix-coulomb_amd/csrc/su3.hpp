@@ -289,6 +289,75 @@ __device__ __forceinline__ void cm_step(mat3<T> &w, mat3<T> &g) {
     }
 }
 
+// The first two steps of the first hit, from g = 1 (what cm_step<0, 1> then
+// cm_step<0, 2> compute there, without the products with the identity's zeros
+// and ones): step (0, 1) sets rows 0, 1 of g to the SU(2) element x, step
+// (0, 2) mixes row 0 = (x00, x01, 0) with row 2 = (0, 0, 1).  The values are
+// those of the general steps (a product with an exact 0 or 1 is exact).
+template <typename T>
+__device__ __forceinline__ void cm_first_two(mat3<T> &w, mat3<T> &g) {
+    m_identity(g);
+    {
+        const cplx<T> waa = w.e[0], wbb = w.e[4], wab = w.e[1], wba = w.e[3];
+        T a0 = waa.re + wbb.re;
+        T a1 = -(wab.im + wba.im);
+        T a2 = -(wab.re - wba.re);
+        T a3 = -(waa.im - wbb.im);
+        T s = a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3;
+        if (s != T(0)) {
+            T r = inv_sqrt(s);
+            a0 *= r;
+            a1 *= r;
+            a2 *= r;
+            a3 *= r;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const cplx<T> p = w.e[c], q = w.e[3 + c];
+                w.e[c] = c_make<T>(a0 * p.re - a3 * p.im + a2 * q.re - a1 * q.im,
+                                   a0 * p.im + a3 * p.re + a2 * q.im + a1 * q.re);
+                w.e[3 + c] = c_make<T>(-a2 * p.re - a1 * p.im + a0 * q.re + a3 * q.im,
+                                       -a2 * p.im + a1 * p.re + a0 * q.im - a3 * q.re);
+            }
+            g.e[0] = c_make<T>(a0, a3);
+            g.e[1] = c_make<T>(a2, a1);
+            g.e[3] = c_make<T>(-a2, a1);
+            g.e[4] = c_make<T>(a0, -a3);
+        }
+    }
+    {
+        const cplx<T> waa = w.e[0], wbb = w.e[8], wab = w.e[2], wba = w.e[6];
+        T a0 = waa.re + wbb.re;
+        T a1 = -(wab.im + wba.im);
+        T a2 = -(wab.re - wba.re);
+        T a3 = -(waa.im - wbb.im);
+        T s = a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3;
+        if (s == T(0)) return;
+        T r = inv_sqrt(s);
+        a0 *= r;
+        a1 *= r;
+        a2 *= r;
+        a3 *= r;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const cplx<T> p = w.e[c], q = w.e[6 + c];
+            w.e[c] = c_make<T>(a0 * p.re - a3 * p.im + a2 * q.re - a1 * q.im,
+                               a0 * p.im + a3 * p.re + a2 * q.im + a1 * q.re);
+            w.e[6 + c] = c_make<T>(-a2 * p.re - a1 * p.im + a0 * q.re + a3 * q.im,
+                                   -a2 * p.im + a1 * p.re + a0 * q.im - a3 * q.re);
+        }
+        // g rows 0 and 2: columns 0, 1 from row 0 alone (row 2 is 0 there),
+        // column 2 from row 2 alone (row 0 is 0 there, row 2 is 1)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const cplx<T> p = g.e[c];
+            g.e[c] = c_make<T>(a0 * p.re - a3 * p.im, a0 * p.im + a3 * p.re);
+            g.e[6 + c] = c_make<T>(-a2 * p.re - a1 * p.im, -a2 * p.im + a1 * p.re);
+        }
+        g.e[2] = c_make<T>(a2, a1);
+        g.e[8] = c_make<T>(a0, -a3);
+    }
+}
+
 // Local Los-Alamos + overrelaxation update from w(n)
 // (gaugefixOverrelaxationLocal / updateLosAlamos / power3x3Binomial,
 // src/gauge_fixing.c:537-708, src/SU3_ops.c:734-771).
@@ -309,13 +378,16 @@ __device__ __forceinline__ void la_or_update(mat3<T> &w, int hits, T omega, mat3
             m_zero(g);
         else
             m_identity(g);
-    } else {
-        m_identity(g);
-        for (int h = 0; h < hits; h++) {
+    } else if (hits > 0) {
+        cm_first_two(w, g);
+        cm_step<T, 1, 2>(w, g);
+        for (int h = 1; h < hits; h++) {
             cm_step<T, 0, 1>(w, g);
             cm_step<T, 0, 2>(w, g);
             cm_step<T, 1, 2>(w, g);
         }
+    } else {
+        m_identity(g);
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
