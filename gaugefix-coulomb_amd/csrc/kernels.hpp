@@ -169,7 +169,7 @@ __device__ __forceinline__ int u0_pivot(const cplx<double> *r0) {
 // row 0 = e[0..2]; row 1 = e[3], e[4] in the two non-pivot columns, and in the
 // pivot column k the element x with sum_j r0_j conj(r1_j) = 0, i.e.
 // x = -conj(S) r0_k / |r0_k|^2, S = sum_{j != k} r0_j conj(r1_j).  |r0_k|^2 is
-// at least 1/3, so the division is well conditioned.
+// at least 1/3, so the division (a reciprocal, rcp_nr) is well conditioned.
 __device__ __forceinline__ cplx<double> csel(bool c, cplx<double> x, cplx<double> y) {
     return c_make<double>(c ? x.re : y.re, c ? x.im : y.im);
 }
@@ -186,7 +186,7 @@ __device__ __forceinline__ void u0_unpack(mat3<double> &m, const cplx<double> *e
     const double sim = __builtin_fma(a.im, s0.re, -__builtin_fma(a.re, s0.im,
                        -__builtin_fma(b.im, s1.re, -(b.re * s1.im))));
     const double nr = __builtin_fma(r.re, r.re, r.im * r.im);
-    const double inv = 1.0 / nr;
+    const double inv = rcp_nr(nr);
     const cplx<double> x = c_make<double>(-__builtin_fma(sre, r.re, sim * r.im) * inv,
                                           -__builtin_fma(sre, r.im, -(sim * r.re)) * inv);
     m.e[0] = e0;
