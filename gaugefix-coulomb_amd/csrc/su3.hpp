@@ -400,7 +400,16 @@ __device__ __forceinline__ void la_or_update(mat3<T> &w, int hits, T omega, mat3
     } else if (hits > 0) {
         cm_first_two(w, g);
         cm_step<T, 1, 2>(w, g);
-        for (int h = 1; h < hits; h++) {
+        for (int h = 1; h < hits - 1; h++) {
+            cm_step<T, 0, 1>(w, g);
+            cm_step<T, 0, 2>(w, g);
+            cm_step<T, 1, 2>(w, g);
+        }
+        // the last hit outside the loop: the entries of w that no later step
+        // reads (row 1's first and row 0's last two columns after (0, 1), rows 0
+        // and 2 but for w21, w22 after (0, 2), everything after (1, 2)) are then
+        // not computed at all
+        if (hits > 1) {
             cm_step<T, 0, 1>(w, g);
             cm_step<T, 0, 2>(w, g);
             cm_step<T, 1, 2>(w, g);
