@@ -268,44 +268,51 @@ __device__ __forceinline__ cplx<T> m_det(const mat3<T> &m) {
     return r;
 }
 
-// One Cabibbo-Marinari step on rows (A, B) (updateSubLosAlamos,
-// src/SU3_ops.c:776-840 with projectSU2, src/SU2_ops.c:369-402): the SU(2)
-// element x = a/|a| with a = (Re(w_AA + w_BB), -Im(w_AB + w_BA),
-// -Re(w_AB - w_BA), -Im(w_AA - w_BB)) maximises Re Tr(x w_sub); rows A, B of
-// w and of the accumulated update g are left-multiplied by
-// [[x0 + i x3, x2 + i x1], [-x2 + i x1, x0 - i x3]].  A null a skips the step.
+// The SU(2) element of a Cabibbo-Marinari step on rows (A, B) of w
+// (updateSubLosAlamos, src/SU3_ops.c:776-840 with projectSU2,
+// src/SU2_ops.c:369-402): x = a/|a| with a = (Re(w_AA + w_BB),
+// -Im(w_AB + w_BA), -Re(w_AB - w_BA), -Im(w_AA - w_BB)) maximises
+// Re Tr(x w_sub).  False for a null a (the step is then skipped).
 template <typename T, int A, int B>
-__device__ __forceinline__ void cm_step(mat3<T> &w, mat3<T> &g) {
+__device__ __forceinline__ bool cm_su2(const mat3<T> &w, T &a0, T &a1, T &a2, T &a3) {
     const cplx<T> waa = w.e[3 * A + A], wbb = w.e[3 * B + B];
     const cplx<T> wab = w.e[3 * A + B], wba = w.e[3 * B + A];
-    T a0 = waa.re + wbb.re;
-    T a1 = -(wab.im + wba.im);
-    T a2 = -(wab.re - wba.re);
-    T a3 = -(waa.im - wbb.im);
-    T s = a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3;
-    if (s == T(0)) return;
-    T r = inv_sqrt(s);
+    a0 = waa.re + wbb.re;
+    a1 = -(wab.im + wba.im);
+    a2 = -(wab.re - wba.re);
+    a3 = -(waa.im - wbb.im);
+    const T s = a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3;
+    if (s == T(0)) return false;
+    const T r = inv_sqrt(s);
     a0 *= r;
     a1 *= r;
     a2 *= r;
     a3 *= r;
-    // x00 = (a0, a3), x01 = (a2, a1), x10 = (-a2, a1), x11 = (a0, -a3)
+    return true;
+}
+
+// Rows A, B of m left-multiplied by the SU(2) element
+// [[x0 + i x3, x2 + i x1], [-x2 + i x1, x0 - i x3]].
+template <typename T, int A, int B>
+__device__ __forceinline__ void cm_rows(mat3<T> &m, T a0, T a1, T a2, T a3) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const cplx<T> p = w.e[3 * A + c], q = w.e[3 * B + c];
-        w.e[3 * A + c] = c_make<T>(a0 * p.re - a3 * p.im + a2 * q.re - a1 * q.im,
+        const cplx<T> p = m.e[3 * A + c], q = m.e[3 * B + c];
+        m.e[3 * A + c] = c_make<T>(a0 * p.re - a3 * p.im + a2 * q.re - a1 * q.im,
                                    a0 * p.im + a3 * p.re + a2 * q.im + a1 * q.re);
-        w.e[3 * B + c] = c_make<T>(-a2 * p.re - a1 * p.im + a0 * q.re + a3 * q.im,
+        m.e[3 * B + c] = c_make<T>(-a2 * p.re - a1 * p.im + a0 * q.re + a3 * q.im,
                                    -a2 * p.im + a1 * p.re + a0 * q.im - a3 * q.re);
     }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const cplx<T> p = g.e[3 * A + c], q = g.e[3 * B + c];
-        g.e[3 * A + c] = c_make<T>(a0 * p.re - a3 * p.im + a2 * q.re - a1 * q.im,
-                                   a0 * p.im + a3 * p.re + a2 * q.im + a1 * q.re);
-        g.e[3 * B + c] = c_make<T>(-a2 * p.re - a1 * p.im + a0 * q.re + a3 * q.im,
-                                   -a2 * p.im + a1 * p.re + a0 * q.im - a3 * q.re);
-    }
+}
+
+// One Cabibbo-Marinari step on rows (A, B): rows A, B of w and of the
+// accumulated update g are left-multiplied by the step's SU(2) element.
+template <typename T, int A, int B>
+__device__ __forceinline__ void cm_step(mat3<T> &w, mat3<T> &g) {
+    T a0, a1, a2, a3;
+    if (!cm_su2<T, A, B>(w, a0, a1, a2, a3)) return;
+    cm_rows<T, A, B>(w, a0, a1, a2, a3);
+    cm_rows<T, A, B>(g, a0, a1, a2, a3);
 }
 
 // The first two steps of the first hit, from g = 1 (what cm_step<0, 1> then
@@ -316,65 +323,26 @@ __device__ __forceinline__ void cm_step(mat3<T> &w, mat3<T> &g) {
 template <typename T>
 __device__ __forceinline__ void cm_first_two(mat3<T> &w, mat3<T> &g) {
     m_identity(g);
-    {
-        const cplx<T> waa = w.e[0], wbb = w.e[4], wab = w.e[1], wba = w.e[3];
-        T a0 = waa.re + wbb.re;
-        T a1 = -(wab.im + wba.im);
-        T a2 = -(wab.re - wba.re);
-        T a3 = -(waa.im - wbb.im);
-        T s = a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3;
-        if (s != T(0)) {
-            T r = inv_sqrt(s);
-            a0 *= r;
-            a1 *= r;
-            a2 *= r;
-            a3 *= r;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const cplx<T> p = w.e[c], q = w.e[3 + c];
-                w.e[c] = c_make<T>(a0 * p.re - a3 * p.im + a2 * q.re - a1 * q.im,
-                                   a0 * p.im + a3 * p.re + a2 * q.im + a1 * q.re);
-                w.e[3 + c] = c_make<T>(-a2 * p.re - a1 * p.im + a0 * q.re + a3 * q.im,
-                                       -a2 * p.im + a1 * p.re + a0 * q.im - a3 * q.re);
-            }
-            g.e[0] = c_make<T>(a0, a3);
-            g.e[1] = c_make<T>(a2, a1);
-            g.e[3] = c_make<T>(-a2, a1);
-            g.e[4] = c_make<T>(a0, -a3);
-        }
+    T a0, a1, a2, a3;
+    if (cm_su2<T, 0, 1>(w, a0, a1, a2, a3)) {
+        cm_rows<T, 0, 1>(w, a0, a1, a2, a3);
+        g.e[0] = c_make<T>(a0, a3);
+        g.e[1] = c_make<T>(a2, a1);
+        g.e[3] = c_make<T>(-a2, a1);
+        g.e[4] = c_make<T>(a0, -a3);
     }
-    {
-        const cplx<T> waa = w.e[0], wbb = w.e[8], wab = w.e[2], wba = w.e[6];
-        T a0 = waa.re + wbb.re;
-        T a1 = -(wab.im + wba.im);
-        T a2 = -(wab.re - wba.re);
-        T a3 = -(waa.im - wbb.im);
-        T s = a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3;
-        if (s == T(0)) return;
-        T r = inv_sqrt(s);
-        a0 *= r;
-        a1 *= r;
-        a2 *= r;
-        a3 *= r;
+    if (!cm_su2<T, 0, 2>(w, a0, a1, a2, a3)) return;
+    cm_rows<T, 0, 2>(w, a0, a1, a2, a3);
+    // g rows 0 and 2: columns 0, 1 from row 0 alone (row 2 is 0 there),
+    // column 2 from row 2 alone (row 0 is 0 there, row 2 is 1)
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const cplx<T> p = w.e[c], q = w.e[6 + c];
-            w.e[c] = c_make<T>(a0 * p.re - a3 * p.im + a2 * q.re - a1 * q.im,
-                               a0 * p.im + a3 * p.re + a2 * q.im + a1 * q.re);
-            w.e[6 + c] = c_make<T>(-a2 * p.re - a1 * p.im + a0 * q.re + a3 * q.im,
-                                   -a2 * p.im + a1 * p.re + a0 * q.im - a3 * q.re);
-        }
-        // g rows 0 and 2: columns 0, 1 from row 0 alone (row 2 is 0 there),
-        // column 2 from row 2 alone (row 0 is 0 there, row 2 is 1)
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const cplx<T> p = g.e[c];
-            g.e[c] = c_make<T>(a0 * p.re - a3 * p.im, a0 * p.im + a3 * p.re);
-            g.e[6 + c] = c_make<T>(-a2 * p.re - a1 * p.im, -a2 * p.im + a1 * p.re);
-        }
-        g.e[2] = c_make<T>(a2, a1);
-        g.e[8] = c_make<T>(a0, -a3);
+    for (int c = 0; c < 2; c++) {
+        const cplx<T> p = g.e[c];
+        g.e[c] = c_make<T>(a0 * p.re - a3 * p.im, a0 * p.im + a3 * p.re);
+        g.e[6 + c] = c_make<T>(-a2 * p.re - a1 * p.im, -a2 * p.im + a1 * p.re);
     }
+    g.e[2] = c_make<T>(a2, a1);
+    g.e[8] = c_make<T>(a0, -a3);
 }
 
 // Local Los-Alamos + overrelaxation update from w(n)
