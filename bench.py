@@ -376,9 +376,9 @@ def main():
                     help="with --profile-kernels: synthesise the input in HBM")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this engine runs one lattice per GPU and the "
-              "boxes here have a single MI355X (see DESIGN.md for the timeslice sharding "
-              "plan); run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the engine fixes a lattice on one MI355X "
+              "(splitting one over several GPUs by timeslices is a design, not built); "
+              "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import gfc
