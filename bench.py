@@ -260,7 +260,7 @@ def io_leg(job, geom, U):
     return out
 
 
-def config5(ctx, ns: int, nt: int, eps: float, seed: int):
+def lattice_64x128(ctx, ns: int, nt: int, eps: float, seed: int):
     """64^3x128 (33.5 M sites): FP64, mixed, and mixed with per-timeslice
     retirement on the reference schedule, plus the GB/s of the FP64 and FP32
     sweep kernels: as the fixes run them (half-sweeps in the default grouping)
@@ -278,7 +278,7 @@ def config5(ctx, ns: int, nt: int, eps: float, seed: int):
         job.generate_smooth(seed, eps)
         job.reunitarize()
         job.save_config()
-        log(f"config5: {ns}^3x{nt} synthesised in HBM in {time.perf_counter() - t:.2f}s")
+        log(f"{ns}^3x{nt}: synthesised in HBM in {time.perf_counter() - t:.2f}s")
         plaq0 = job.plaquette()
         runs = {"fp64": gfc.Options(precision=gfc.PREC_FP64),
                 "mixed": gfc.Options(precision=gfc.PREC_MIXED),
@@ -298,7 +298,7 @@ def config5(ctx, ns: int, nt: int, eps: float, seed: int):
                          "plaquette_drift": max(abs(plaq[0] - plaq0[0]),
                                                 abs(plaq[1] - plaq0[1])),
                          "site_updates_per_s": rep.slice_sweeps * vh / dt}
-            log(f"config5 {name}: {dt:.3f}s sweeps={rc} fp32={rep.sweeps_fp32} "
+            log(f"{ns}^3x{nt} {name}: {dt:.3f}s sweeps={rc} fp32={rep.sweeps_fp32} "
                 f"slice-sweeps={rep.slice_sweeps:.1f} e2={e2:.3e}")
         out["mixed_speedup_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
         out["mixed_retire_speedup_vs_fp64"] = round(
@@ -323,7 +323,7 @@ def config5(ctx, ns: int, nt: int, eps: float, seed: int):
             out[f"fix_{name}_traffic_bytes_per_update"] = t_upd
             if t_upd:
                 out[f"fix_{name}_traffic_GBs"] = round(t_upd * vh / (ms * 1e-3) / 1e9, 1)
-            log(f"config5 {name} fix sweeps: {ms:.4f} ms per half-sweep")
+            log(f"{ns}^3x{nt} {name} fix sweeps: {ms:.4f} ms per half-sweep")
         fixed = gfc.Params(tolerance=1e-300, estimate_sweeps_to_gf_progress=10 ** 8,
                            max_sweeps_to_fix=40, sweeps_to_reunitarization=10 ** 6)
         for name, opt, bpu in (("fp64", gfc.PREC_FP64, ALG_BYTES_PER_UPDATE_FP64),
@@ -341,7 +341,7 @@ def config5(ctx, ns: int, nt: int, eps: float, seed: int):
             out[f"{name}_sweep_hbm_bytes_per_launch"] = traffic
             if traffic:
                 out[f"{name}_sweep_traffic_GBs"] = round(traffic / (ms * 1e-3) / 1e9, 1)
-            log(f"config5 {name} sweep kernel: {ms:.4f} ms/launch, {gbs:.1f} GB/s algorithmic")
+            log(f"{ns}^3x{nt} {name} sweep kernel: {ms:.4f} ms/launch, {gbs:.1f} GB/s algorithmic")
     return out
 
 
@@ -358,7 +358,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-io", action="store_true", help="skip the file-load leg")
-    ap.add_argument("--no-config5", action="store_true",
+    ap.add_argument("--no-64x128", action="store_true",
                     help="skip the 64^3x128 mixed/retirement leg")
     ap.add_argument("--serve", type=int, default=256, metavar="N",
                     help="requests in the serving stream (0: skip)")
@@ -572,13 +572,13 @@ def main():
         serve_out["by_size"] = {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv)
                                     for kk, vv in v.items()}
                                 for k, v in serve_out["by_size"].items()}
-        serve_out["workload"] = ("config 4: sizes {16^3x32: 40%, 24^3x48: 30%, 32^3x64: 20%, "
+        serve_out["workload"] = ("serving mix: sizes {16^3x32: 40%, 24^3x48: 30%, 32^3x64: 20%, "
                                  "48^3x96: 10%}, eps U[0.2,0.4], tol {1e-14, 1e-16}, "
                                  f"Poisson {a.serve_rate}/s, inputs synthesised in HBM")
         log(f"serve: {serve_out}")
     c5 = None
-    if not a.no_config5:
-        c5 = config5(ctx, 64, 128, a.eps, a.seed)
+    if not a.no_64x128:
+        c5 = lattice_64x128(ctx, 64, 128, a.eps, a.seed)
     ctx.close()
 
     cpu = None
@@ -641,7 +641,7 @@ def main():
             if traffic_upd else None},
         "copy_kernel_GBs": round(copy_gbs, 1),
         "serve": serve_out,
-        "config5": c5,
+        "lattice_64x128": c5,
         "io": io,
         "cpu_baseline": cpu,
     }

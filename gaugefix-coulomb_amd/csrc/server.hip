@@ -362,8 +362,8 @@ int next_multiple(int s, int m) { return (s / m + 1) * m; }
 // half-sweeps) in timeslice waves on the lane's streams.  The lane serves its
 // lattices first come, first served: a lattice admitted behind another waits
 // until that one is done, rather than sharing the lane with it chunk by chunk,
-// which made both late (config-4 stream: p99 1.035 -> 0.991 s, mean latency
-// 173 -> 166 ms).
+// which made both late (the 256-request serving stream: p99 1.035 -> 0.991 s,
+// mean latency 173 -> 166 ms).
 int run_waves(gfc_server *s, Lane &L) {
     Req *run = nullptr;
     for (auto &a : L.active)  // in order of admission

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import Geometry, Job, Options, Params, PREC_MIXED, Server
 
-CONFIG4_SIZES = ((16, 32, 0.4), (24, 48, 0.3), (32, 64, 0.2), (48, 96, 0.1))
+SERVING_MIX_SIZES = ((16, 32, 0.4), (24, 48, 0.3), (32, 64, 0.2), (48, 96, 0.1))
 
 
 @dataclass
@@ -31,7 +31,7 @@ class Request:
     tolerance: float
 
 
-def stream_requests(n=256, rate=4.0, sizes=CONFIG4_SIZES, eps_range=(0.2, 0.4),
+def stream_requests(n=256, rate=4.0, sizes=SERVING_MIX_SIZES, eps_range=(0.2, 0.4),
                     tolerances=(1e-14, 1e-16), seed=4, base_seed=1000):
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=n)

@@ -18,11 +18,11 @@ reference's code.  Outputs:
                    initial reunitarization, the reunitarized U, e2/theta/F/
                    plaquettes after k = 1..200 half-sweeps, G after k in
                    KEEP_G, U after 200, runs to tolerance (estimate 1000 and 0)
-  lat_8x16.npz     config 1 (8^3x16, eps 0.25, seed 2): observables after
+  lat_8x16.npz     8^3x16, eps 0.25, seed 2: observables after
                    k = 1..200, G (sampled sites) after 200, runs to tolerance
   lat_2x2.npz      2^3x2 (eps 0.3, seed 11) and 6^3x4 (eps 0.3, seed 12),
   lat_6x4.npz      the same records as lat_4x8.npz
-  lat_32x64.npz    config 2 (32^3x64, eps 0.3, seed 3): observables after
+  lat_32x64.npz    32^3x64, eps 0.3, seed 3: observables after
                    every half-sweep k = 1..200 (harness mode `trace`, checked
                    bit for bit against `traj` on the small lattices), G
                    (sampled sites) after 200, run to tolerance

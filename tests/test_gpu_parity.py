@@ -139,8 +139,8 @@ def test_drop_in_host_arrays(ctx):
     assert np.max(np.abs(U - f["U_200"])) < 1e-12
 
 
-def test_config2_32x64(ctx, tmp_path):
-    """Config 2: 32^3x64, eps 0.3, read from a little-endian FP64 SCIDAC file
+def test_32x64_from_le_fp64_file(ctx, tmp_path):
+    """32^3x64, eps 0.3, read from a little-endian FP64 SCIDAC file
     (streamed into HBM) -- e2 after every half-sweep k = 1..200, G after 200,
     plaquette invariance and the sweep count to 1e-16 against the reference."""
     f = golden("lat_32x64.npz")

@@ -53,7 +53,7 @@ def test_retirement_needs_ns_multiple_of_8(ctx):
 
 @pytest.mark.parametrize("precision", [gfc.PREC_FP64, gfc.PREC_MIXED])
 def test_retirement_against_reference(ctx, precision):
-    """Slice retirement on config 1 (8^3x16, the reference's fixture run to
+    """Slice retirement on 8^3x16 (eps 0.25, seed 2: the reference's fixture run to
     1e-16 under its default schedule): the same sweep count, e2 <= 1e-16 and
     the reference's gauge functional F to 1e-10 relative (retired slices stop
     moving below the tolerance, so G is not compared)."""

@@ -200,8 +200,8 @@ def test_served_null_row_fails(ctx):
             assert np.array_equal(srv.result_gauge(comps[1].id, geom), job.download_gauge())
 
 
-def test_config4_stream_alone_equals_in_the_mix(ctx):
-    """A config-4-shaped stream: 8 requests with the serving workload's sizes
+def test_serving_mix_stream_alone_equals_in_the_mix(ctx):
+    """A stream shaped like the serving workload: 8 requests with the serving workload's sizes
     (16^3x32 ... 48^3x96, one of each at least), eps in [0.2, 0.4], tolerances
     1e-14 and 1e-16, Poisson arrivals at 4/s, mixed precision.  Each served G
     equals its lone gfc_job_fix bit for bit, with the same sweep counts."""

@@ -1,4 +1,4 @@
-"""Host logic of the serving workload (gfc.serve): the config-4 request stream,
+"""Host logic of the serving workload (gfc.serve): the request stream of the serving mix,
 the latency summary and the offered load rho = lambda E[S], on synthetic
 completions (no GPU)."""
 from types import SimpleNamespace
@@ -9,14 +9,14 @@ import pytest
 serve = pytest.importorskip("gfc.serve")
 
 
-def test_stream_requests_follow_config4():
+def test_stream_requests_follow_the_serving_mix():
     reqs = serve.stream_requests(n=4000, rate=4.0)
     t = np.array([r.t_arrival for r in reqs])
     assert t[0] == 0.0 and np.all(np.diff(t) >= 0)
     # Poisson arrivals at 4/s: mean gap 0.25 s
     assert abs(np.diff(t).mean() - 0.25) < 0.02
     sizes = [(r.geom.ns, r.geom.nt) for r in reqs]
-    for ns, nt, w in serve.CONFIG4_SIZES:
+    for ns, nt, w in serve.SERVING_MIX_SIZES:
         assert abs(sizes.count((ns, nt)) / len(reqs) - w) < 0.03
     assert all(0.2 <= r.eps <= 0.4 for r in reqs)
     assert {r.tolerance for r in reqs} == {1e-14, 1e-16}

@@ -134,8 +134,8 @@ def test_full_run_sweep_count(name, est):
 
 
 @pytest.mark.slow
-def test_trajectory_8x16_config1():
-    """Config 1 (8^3x16): e2 after each of 200 half-sweeps, G at 200, and the
+def test_trajectory_8x16():
+    """8^3x16 (eps 0.25, seed 2): e2 after each of 200 half-sweeps, G at 200, and the
     sweep counts to tolerance under the default and the dense schedule."""
     f = golden("lat_8x16.npz")
     geom, U0 = _config(f)
